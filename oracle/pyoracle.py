@@ -1,0 +1,167 @@
+"""ctypes binding of the CPU oracle (test infrastructure only).
+
+Only the tests, the smoke check and the CPU-baseline leg of bench.py use this
+module, and only as the checker.  Arrays are numpy float64 in the SPADE
+reference layout, shape (nblocks, Nz, Ny, Nx, 5) with ghosts stored
+(core/mem_map.h:466-497, 654-676).
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+MAP_IDENTITY, MAP_TANH = 0, 1
+TERM_KEEP, TERM_TOTANI, TERM_PDISS, TERM_WENO, TERM_FWENO, TERM_FIRST, TERM_VISC = range(1, 8)
+SENSOR_CONST, SENSOR_DUCROS = 0, 1
+SPLIT_RUSANOV, SPLIT_PHYS = 0, 1
+LAW_CONST, LAW_POWER = 0, 1
+MAX_TERMS = 4
+
+
+class Map1d(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p0", C.c_double), ("p1", C.c_double), ("beta", C.c_double)]
+
+
+class Grid(C.Structure):
+    _fields_ = [("nb", C.c_int * 3), ("nc", C.c_int * 3), ("ng", C.c_int),
+                ("lo", C.c_double * 3), ("hi", C.c_double * 3), ("periodic", C.c_int * 3),
+                ("map", Map1d * 3)]
+
+
+class Gas(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("R", C.c_double)]
+
+
+class Term(C.Structure):
+    _fields_ = [("kind", C.c_int), ("order", C.c_int), ("sensor", C.c_int),
+                ("sensor_param", C.c_double), ("eps_p", C.c_double), ("eps_T", C.c_double),
+                ("split", C.c_int), ("smooth", C.c_int), ("law", C.c_int),
+                ("mu", C.c_double), ("T_ref", C.c_double), ("power", C.c_double),
+                ("prandtl", C.c_double)]
+
+
+class Flux(C.Structure):
+    _fields_ = [("nterms", C.c_int), ("terms", Term * MAX_TERMS)]
+
+
+class BC(C.Structure):
+    _fields_ = [("wall_mask", C.c_int), ("t_wall", C.c_double)]
+
+
+def _load():
+    path = os.path.join(_HERE, "liboracle.so")
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"oracle library not built: {path} (run make -C oracle)")
+    lib = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    lib.ora_array_size.argtypes = [C.POINTER(Grid)]
+    lib.ora_flux_div.argtypes = [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Flux), dp, dp, C.c_int]
+    lib.ora_exchange.argtypes = [C.POINTER(Grid), dp]
+    lib.ora_boundary_fill_wall.argtypes = [C.POINTER(Grid), dp, C.c_int, C.c_double]
+    lib.ora_prim_to_cons.argtypes = [C.POINTER(Gas), dp, dp, C.c_long]
+    lib.ora_cons_to_prim.argtypes = [C.POINTER(Gas), dp, dp, C.c_long]
+    lib.ora_ssprk3_step.argtypes = [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Flux), C.POINTER(BC),
+                                    C.c_double, dp, dp, dp, dp]
+    lib.ora_ssprk3opt_step.argtypes = [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Flux), C.POINTER(BC),
+                                       C.c_double, dp, dp, dp]
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def make_grid(nb, nc, ng, lo, hi, periodic=(1, 1, 1), maps=None):
+    g = Grid()
+    for d in range(3):
+        g.nb[d], g.nc[d] = nb[d], nc[d]
+        g.lo[d], g.hi[d] = lo[d], hi[d]
+        g.periodic[d] = int(periodic[d])
+        m = (maps or [None] * 3)[d]
+        if m is None:
+            g.map[d].kind = MAP_IDENTITY
+        else:
+            g.map[d].kind, g.map[d].p0, g.map[d].p1, g.map[d].beta = m
+    g.ng = ng
+    return g
+
+
+def make_term(kind, **kw):
+    t = Term()
+    t.kind = kind
+    for k, v in kw.items():
+        setattr(t, k, v)
+    return t
+
+
+def make_flux(*terms):
+    f = Flux()
+    f.nterms = len(terms)
+    for i, t in enumerate(terms):
+        f.terms[i] = t
+    return f
+
+
+def shape_of(g):
+    nblk = g.nb[0] * g.nb[1] * g.nb[2]
+    return (nblk, g.nc[2] + 2 * g.ng, g.nc[1] + 2 * g.ng, g.nc[0] + 2 * g.ng, 5)
+
+
+def _ptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def flux_div(g, gas, flux, prim, rhs=None, increment=False):
+    if rhs is None:
+        rhs = np.zeros(shape_of(g))
+    rc = lib().ora_flux_div(C.byref(g), C.byref(gas), C.byref(flux), _ptr(prim), _ptr(rhs), int(increment))
+    if rc != 0:
+        raise RuntimeError("ora_flux_div failed")
+    return rhs
+
+
+def exchange(g, a):
+    lib().ora_exchange(C.byref(g), _ptr(a))
+    return a
+
+
+def boundary_fill_wall(g, a, face, t_wall):
+    lib().ora_boundary_fill_wall(C.byref(g), _ptr(a), face, t_wall)
+    return a
+
+
+def prim_to_cons(gas, prim):
+    out = np.empty_like(prim)
+    lib().ora_prim_to_cons(C.byref(gas), _ptr(prim), _ptr(out), prim.size // 5)
+    return out
+
+
+def cons_to_prim(gas, cons):
+    out = np.empty_like(cons)
+    lib().ora_cons_to_prim(C.byref(gas), _ptr(cons), _ptr(out), cons.size // 5)
+    return out
+
+
+def ssprk3_step(g, gas, flux, bc, dt, q, nsteps=1):
+    r = [np.zeros(shape_of(g)) for _ in range(3)]
+    for _ in range(nsteps):
+        lib().ora_ssprk3_step(C.byref(g), C.byref(gas), C.byref(flux), C.byref(bc) if bc else None,
+                              dt, _ptr(q), _ptr(r[0]), _ptr(r[1]), _ptr(r[2]))
+    return q
+
+
+def ssprk3opt_step(g, gas, flux, bc, dt, q, nsteps=1):
+    r = [np.zeros(shape_of(g)) for _ in range(2)]
+    for _ in range(nsteps):
+        lib().ora_ssprk3opt_step(C.byref(g), C.byref(gas), C.byref(flux), C.byref(bc) if bc else None,
+                                 dt, _ptr(q), _ptr(r[0]), _ptr(r[1]))
+    return q

@@ -1,0 +1,68 @@
+/* C interface of the CPU oracle (test infrastructure only; see
+ * spade_oracle.cpp).  Arrays are in the SPADE reference layout: AoS with the
+ * variable index fastest and ghost layers stored, blocks x-fastest. */
+#pragma once
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORA_MAP_IDENTITY = 0, ORA_MAP_TANH = 1 };
+
+typedef struct
+{
+    int kind;           /* ORA_MAP_* */
+    double p0, p1;      /* computational interval the map is stretched over */
+    double beta;        /* tanh stretching factor */
+} ora_map1d;
+
+typedef struct
+{
+    int nb[3];          /* blocks per axis */
+    int nc[3];          /* cells per block per axis */
+    int ng;             /* ghost layers */
+    double lo[3], hi[3];
+    int periodic[3];
+    ora_map1d map[3];
+} ora_grid;
+
+typedef struct { double gamma, R; } ora_gas;
+
+enum { ORA_TERM_KEEP = 1, ORA_TERM_TOTANI, ORA_TERM_PDISS, ORA_TERM_WENO,
+       ORA_TERM_FWENO, ORA_TERM_FIRST, ORA_TERM_VISC };
+enum { ORA_SENSOR_CONST = 0, ORA_SENSOR_DUCROS = 1 };
+enum { ORA_SPLIT_RUSANOV = 0, ORA_SPLIT_PHYS = 1 };
+enum { ORA_LAW_CONST = 0, ORA_LAW_POWER = 1 };
+#define ORA_MAX_TERMS 4
+
+typedef struct
+{
+    int kind;
+    int order;                  /* KEEP */
+    int sensor;                 /* PDISS */
+    double sensor_param;        /* const alpha, or Ducros epsilon */
+    double eps_p, eps_T;        /* PDISS */
+    int split;                  /* WENO */
+    int smooth;                 /* WENO, FWENO: 1 = nonlinear weights */
+    int law;                    /* VISC */
+    double mu, T_ref, power, prandtl;
+} ora_term;
+
+typedef struct { int nterms; ora_term terms[ORA_MAX_TERMS]; } ora_flux;
+
+typedef struct { int wall_mask; double t_wall; } ora_bc;
+
+int ora_array_size(const ora_grid* g);
+int ora_flux_div(const ora_grid* g, const ora_gas* gas, const ora_flux* fl,
+                 const double* prim, double* rhs, int increment);
+int ora_exchange(const ora_grid* g, double* a);
+int ora_boundary_fill_wall(const ora_grid* g, double* a, int face, double t_wall);
+int ora_prim_to_cons(const ora_gas* gas, const double* prim, double* cons, long ncell);
+int ora_cons_to_prim(const ora_gas* gas, const double* cons, double* prim, long ncell);
+int ora_ssprk3_step(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, const ora_bc* bc,
+                    double dt, double* q, double* r0, double* r1, double* r2);
+int ora_ssprk3opt_step(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, const ora_bc* bc,
+                       double dt, double* q, double* r0, double* r1);
+
+#ifdef __cplusplus
+}
+#endif
