@@ -1,0 +1,185 @@
+"""Benchmark of the SPADE right-hand-side hot path on one MI355X.
+
+Default workload: FP64 flux_div (cent_keep<2> + visc_lr with the power law)
+on the 256^3 Taylor-Green vortex, 16x16x16 blocks of 16^3 cells with 2 ghost
+layers, inputs resident in HBM.  One step = one flux_div over the whole grid.
+Prints one JSON line (value = Mcell-RHS/s).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL_RHS = 80.0   # read 5 x 8 B primitives + write 5 x 8 B residual
+
+
+def tgv_interior(nb, nc, L, p0):
+    """Analytic TGV primitives, shape (blocks, nz, ny, nx, 5), blocks x-fastest."""
+    n = [nb[d] * nc[d] for d in range(3)]
+    dx = [L / n[d] for d in range(3)]
+    x = (np.arange(n[0]) + 0.5) * dx[0]
+    y = (np.arange(n[1]) + 0.5) * dx[1]
+    z = (np.arange(n[2]) + 0.5) * dx[2]
+    # (bz, k, by, j, bx, i) views of the global axes
+    X = x.reshape(1, 1, 1, 1, nb[0], nc[0])
+    Y = y.reshape(1, 1, nb[1], nc[1], 1, 1)
+    Z = z.reshape(nb[2], nc[2], 1, 1, 1, 1)
+    shape = (nb[2], nc[2], nb[1], nc[1], nb[0], nc[0])
+    out = np.empty((nb[2], nb[1], nb[0], nc[2], nc[1], nc[0], 5))
+
+    def put(v, field):
+        f = np.broadcast_to(field, shape)
+        out[..., v] = f.transpose(0, 2, 4, 1, 3, 5)
+
+    put(0, p0 + (1.0 / 16.0) * (np.cos(2 * X) + np.cos(2 * Y)) * (np.cos(2 * Z) + 2.0))
+    put(1, np.ones((1, 1, 1, 1, 1, 1)))
+    put(2, np.sin(X) * np.cos(Y) * np.cos(Z))
+    put(3, -np.cos(X) * np.sin(Y) * np.cos(Z))
+    put(4, np.zeros((1, 1, 1, 1, 1, 1)))
+    return out.reshape(nb[0] * nb[1] * nb[2], nc[2], nc[1], nc[0], 5)
+
+
+def cpu_baseline(sample_blocks=4, reps=5):
+    """The compiled reference's own CPU flux_div (oracle/_ref), one core, on a
+    bounded sample of the same workload (TGV, 16^3 blocks, same flux)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if os.path.exists(exe):
+        try:
+            r = subprocess.run([exe, "--bench", str(sample_blocks), str(reps)], capture_output=True, text=True,
+                               timeout=300, check=True)
+            line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
+            d = json.loads(line)
+            return {"value": d["mcell_rhs_per_s"], "unit": "Mcell-RHS/s", "cores": 1, "kind": "reference",
+                    "sample": f"TGV {sample_blocks}^3 blocks of 16^3 ({int(d['cells'])} cells), cent_keep<2>+visc_lr "
+                              f"power law, flux_div basic, median of {reps}",
+                    "cpu": _cpu_model()}
+        except Exception as e:  # pragma: no cover - reported, then the port is timed
+            print(f"reference baseline unavailable: {e}", file=sys.stderr)
+    from oracle import pyoracle as po
+    nb, nc, ng = (sample_blocks,) * 3, (16, 16, 16), 2
+    L = 2.0 * np.pi
+    p0 = 100.0 / 1.4
+    g = po.make_grid(nb, nc, ng, (0, 0, 0), (L, L, L))
+    q = np.zeros(po.shape_of(g))
+    q[:, ng:-ng, ng:-ng, ng:-ng] = tgv_interior(nb, nc, L, p0)
+    po.exchange(g, q)
+    fl = po.make_flux(po.make_term(po.TERM_KEEP, order=2),
+                      po.make_term(po.TERM_VISC, law=po.LAW_POWER, mu=1 / 1600, T_ref=1.0, power=0.76, prandtl=0.72))
+    gas = po.Gas(1.4, p0)
+    po.flux_div(g, gas, fl, q)
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        po.flux_div(g, gas, fl, q)
+        ts.append(time.perf_counter() - t0)
+    cells = np.prod(nb) * 4096
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": cells / sorted(ts)[len(ts) // 2] * 1e-6, "unit": "Mcell-RHS/s", "cores": threads,
+            "kind": "port", "sample": f"TGV {sample_blocks}^3 blocks of 16^3, oracle flux_div, median of {reps}",
+            "cpu": _cpu_model()}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _traffic(workload):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+    (profiles/traffic.json), or None when not measured."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as fh:
+            return json.load(fh).get(workload)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--blocks", type=int, default=16, help="blocks per axis (16 -> 256^3)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by blocks "
+              "across GPUs but no multi-GPU run is part of this build", file=sys.stderr)
+        sys.exit(2)
+
+    import spade_amd as sa
+    sa.load()
+    nb, nc, ng = (args.blocks,) * 3, (16, 16, 16), 2
+    L = 2.0 * np.pi
+    gamma, R = 1.4, 100.0 / 1.4
+    p0 = R
+    t0 = time.perf_counter()
+    g = sa.CartesianGrid(nc, nb, ((0.0, L),) * 3, num_exch=ng)
+    host = np.zeros(g.host_shape)
+    host[:, ng:-ng, ng:-ng, ng:-ng] = tgv_interior(nb, nc, L, p0)
+    q = sa.GridArray(g).upload(host)
+    del host
+    sa.create_exchange(q).exchange(q)
+    rhs = sa.GridArray(g, fill=0.0)
+    gas = sa.IdealGas(gamma, R)
+    flux = sa.compose(sa.cent_keep(2, gas), sa.visc_lr(sa.power_law(1.0 / 1600.0, 1.0, 0.76, 0.72), gas))
+    g.ctx.sync()
+    print(f"setup {time.perf_counter() - t0:.1f}s, {g.num_cells} cells", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        sa.flux_div(q, rhs, flux, increment=False)
+    g.ctx.sync()
+    g.ctx.timer_start()
+    for _ in range(args.steps):
+        sa.flux_div(q, rhs, flux, increment=False)
+    ms_total = g.ctx.timer_stop()
+    g.ctx.sync()
+    ms = ms_total / args.steps
+    cells = g.num_cells
+    value = cells / (ms * 1e-3) * 1e-6
+    achieved = BYTES_PER_CELL_RHS * cells / (ms * 1e-3) / 1e9
+    workload = f"tgv{nb[0] * nc[0]}_fp64_flux_div"
+    traffic = _traffic(workload)
+    line = {
+        "metric": "Mcell-RHS/s (FP64 flux_div, cent_keep<2> + visc_lr power law)",
+        "value": round(value, 2),
+        "unit": "Mcell-RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (analytic Taylor-Green initial state)",
+        "config": {"workload": f"TGV {nb[0] * nc[0]}^3, {nb[0]}^3 blocks of 16^3, ng=2, one flux_div per step",
+                   "cells": cells},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+    }
+    if not args.no_cpu_baseline:
+        cb = cpu_baseline()
+        line["cpu_baseline"] = cb
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,153 @@
+/* spade_amd: MI355X-native right-hand-side hot path of SPADE.
+ *
+ * C ABI of libspade_amd.so.  Plain pointers, sizes and status codes only.
+ * Every entry point returns SPD_OK (0) or an SPD_ERR_* code; the message of
+ * the last failure on the calling thread is spd_last_error().
+ *
+ * Host arrays passed to upload/download use the reference storage layout:
+ * AoS with the variable index fastest, ghost layers stored, blocks in x-fastest
+ * order:  offset = v + 5*((i+ng) + Nx*((j+ng) + Ny*((k+ng) + Nz*lb))),
+ * N* = n* + 2 ng  (SPADE core/mem_map.h:466-497, 654-676).
+ * Primitive order (p, T, u, v, w); flux/residual order (continuity, energy,
+ * x-, y-, z-momentum) (navier-stokes/fluid_state.h:11-77).
+ */
+#ifndef SPADE_AMD_H
+#define SPADE_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    SPD_OK = 0,
+    SPD_ERR_ARG = 1,          /* bad argument / size / configuration (reference: except::sp_exception, core/except.h:12-30) */
+    SPD_ERR_HIP = 2,          /* HIP runtime or launch failure (reference: CUDA error check after every launch, dispatch/execute.h:85-96) */
+    SPD_ERR_UNSUPPORTED = 3,  /* combination the library does not implement */
+    SPD_ERR_MEMORY = 4        /* device allocation failure (reference: mem_exception, dispatch/device_allocator.h:30-44) */
+};
+
+typedef struct spd_ctx_s*        spd_ctx_t;
+typedef struct spd_grid_s*       spd_grid_t;
+typedef struct spd_array_s*      spd_array_t;
+typedef struct spd_integrator_s* spd_integrator_t;
+
+/* Message of the last failure on this thread ("" if none). */
+const char* spd_last_error(void);
+
+/* ---- context: one device, one HIP stream; every call is stream-ordered ---- */
+int spd_ctx_create(int device, spd_ctx_t* out);
+int spd_ctx_destroy(spd_ctx_t ctx);
+int spd_ctx_sync(spd_ctx_t ctx);
+/* The hipStream_t all work of this context is issued on. */
+void* spd_ctx_stream(spd_ctx_t ctx);
+/* HIP-event timer around work on the context stream (milliseconds). */
+int spd_timer_start(spd_ctx_t ctx);
+int spd_timer_stop(spd_ctx_t ctx, float* ms);
+
+/* ---- grid: cartesian_grid_t + cartesian_blocks_t
+ *      (grid/cartesian_grid.h:84-96, grid/cartesian_blocks.h:38-107) ---- */
+enum { SPD_MAP_IDENTITY = 0, SPD_MAP_TANH = 1 };
+typedef struct {
+    int    kind;        /* SPD_MAP_* */
+    double p0, p1;      /* computational interval the map stretches over */
+    double beta;        /* tanh stretching factor: x = c + h tanh(beta s)/tanh(beta) */
+} spd_map1d;
+
+typedef struct {
+    int       num_blocks[3];
+    int       cells_in_block[3];
+    int       num_exch;          /* ghost layers (same on every axis) */
+    double    lo[3], hi[3];      /* computational bounds */
+    int       periodic[3];
+    spd_map1d map[3];            /* identity or diagonal (1-D per axis) coordinates */
+} spd_grid_desc;
+
+int spd_grid_create(spd_ctx_t ctx, const spd_grid_desc* desc, spd_grid_t* out);
+int spd_grid_destroy(spd_grid_t grid);
+
+/* ---- grid_array: 5-variable FP64 cell-centred array with ghosts
+ *      (grid/grid_array.h:182-422).  Device layout is block-major SoA
+ *      [block][var][k][j][i] with ghosts. ---- */
+int spd_array_create(spd_grid_t grid, spd_array_t* out);
+int spd_array_destroy(spd_array_t arr);
+int spd_array_upload(spd_array_t arr, const double* host);    /* reference layout */
+int spd_array_download(spd_array_t arr, double* host);        /* reference layout */
+/* arr = value on every stored cell (grid_array::operator=(scalar), grid_array.h:289-395) */
+int spd_array_fill(spd_array_t arr, double value);
+int spd_array_copy(spd_array_t dst, spd_array_t src);
+/* Number of doubles of the reference-layout host image of an array. */
+long spd_array_host_size(spd_grid_t grid);
+
+/* ---- flux functors (composed as F0 + (F1 + ...), omni/compose.h:7-45) ---- */
+enum {
+    SPD_TERM_KEEP = 1,    /* cent_keep<order>            navier-stokes/convective.h:96-192 */
+    SPD_TERM_TOTANI,      /* totani_lr                   convective.h:54-94 */
+    SPD_TERM_PDISS,       /* pressure_diss_lr            convective.h:194-246 */
+    SPD_TERM_WENO,        /* weno_t<rusanov|phys_flux>   convective.h:255-333, flux_funcs.h */
+    SPD_TERM_FWENO,       /* fweno_t                     convective.h:335-497 */
+    SPD_TERM_FIRST,       /* first_order_t (rusanov)     convective.h:14-33 */
+    SPD_TERM_VISC         /* visc_lr                     navier-stokes/viscous.h:13-113 */
+};
+enum { SPD_SENSOR_CONST = 0, SPD_SENSOR_DUCROS = 1 };   /* state_sensor.h:21-60 */
+enum { SPD_SPLIT_RUSANOV = 0, SPD_SPLIT_PHYS = 1 };     /* flux_funcs.h:8-126 */
+enum { SPD_LAW_CONST = 0, SPD_LAW_POWER = 1 };          /* viscous_laws.h:62-135 */
+#define SPD_MAX_TERMS 4
+
+typedef struct {
+    int    kind;
+    int    order;                    /* KEEP: even nominal order (2 or 4 with 2 ghosts) */
+    int    sensor;                   /* PDISS */
+    double sensor_param;             /* const: alpha; Ducros: epsilon */
+    double eps_p, eps_T;             /* PDISS */
+    int    split;                    /* WENO */
+    int    smooth;                   /* WENO/FWENO: 1 = nonlinear weights (enable_smooth) */
+    int    law;                      /* VISC */
+    double mu, T_ref, power, prandtl;/* VISC: constant mu, or power law mu_ref (T/T_ref)^power */
+} spd_flux_term;
+
+typedef struct {
+    int           nterms;
+    spd_flux_term terms[SPD_MAX_TERMS];
+    double        gamma, R;          /* ideal_gas_t (navier-stokes/gas.h:35-49) */
+} spd_flux_desc;
+
+enum { SPD_OVERWRITE = 0, SPD_INCREMENT = 1 };   /* pde-algs/pde_traits.h:7-20 */
+
+/* rhs (+)= -div F on interior cells: pde_algs::flux_div(prims, rhs, flux, traits)
+ * (pde-algs/flux-div/flux_div.h:16-56, flux_div_basic.h:17-76).  prims must have
+ * current ghosts.  Overwrite mode also zeroes the ghosts of rhs, as the
+ * reference's rhs = 0 does. */
+int spd_flux_div(spd_array_t prims, spd_array_t rhs, const spd_flux_desc* flux, int mode);
+
+/* Ghost exchange between blocks (create_exchange + exchange,
+ * grid/exchange_pattern.h:304-424): periodic axes wrap, domain faces of
+ * non-periodic axes are left to the boundary fill. */
+int spd_exchange(spd_array_t arr);
+
+/* boundary_fill with the no-slip isothermal mirror kernel
+ * ghost = (p, 2 T_wall - T, -u, -v, -w) of cell -1-i / 2n-(i+1)
+ * (grid/boundary_fill.h:32-133).  face = 2*dir + (0: min, 1: max). */
+int spd_boundary_fill_wall(spd_array_t arr, int face, double t_wall);
+
+/* ---- time integration: integrator_t(axis, scheme, data, rhs, bc, trans).advance()
+ *      (time-integration/integrator.h:18-59) with rhs = flux_div(overwrite),
+ *      bc = exchange + walls, trans = prim<->cons state transform. ---- */
+enum { SPD_SCHEME_SSPRK3 = 0,       /* ssprk3_t, transform-aware advance, advance.h:233-280 */
+       SPD_SCHEME_SSPRK3_OPT = 1 }; /* ssprk3_opt low storage, advance.h:354-402 */
+typedef struct { int wall_mask; double t_wall; } spd_bc_desc;  /* bit (2*dir+pm): isothermal wall */
+
+int spd_integrator_create(spd_array_t q, const spd_flux_desc* flux, const spd_bc_desc* bc,
+                          int scheme, double dt, spd_integrator_t* out);
+int spd_integrator_advance(spd_integrator_t it, int nsteps);
+int spd_integrator_destroy(spd_integrator_t it);
+/* Residual array k_i of the integrator (0-based), for inspection. */
+spd_array_t spd_integrator_residual(spd_integrator_t it, int i);
+
+/* ---- reductions on interior cells (transform_reduce, algs/transform_reduce.h:53-190) ---- */
+/* max(|u| + sqrt(gamma R T)): the CFL wave speed (development/cuda-tgv/main.cc:155-172) */
+int spd_max_wavespeed(spd_array_t prims, double gamma, double R, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -17,6 +17,8 @@
 #include <string>
 #include <vector>
 #include <fstream>
+#include <chrono>
+#include <algorithm>
 #include "spade.h"
 
 using real_t = double;
@@ -92,9 +94,62 @@ static real_t rough(int i, int j, int k, int v)
     return 2.0*(s - std::floor(s)) - 1.0;
 }
 
+// Timing mode: flux_div (basic) with cent_keep<2> + visc_lr(power law) on a
+// TGV grid of nb^3 blocks of 16^3 cells, repeated reps times on one core.
+// Prints one JSON line with the throughput in Mcell-RHS/s.
+static int run_bench(spade::parallel::mpi_t& group, int nb, int reps)
+{
+    const real_t gamma = 1.4, mach = 0.1, u0 = 1.0, rho0 = 1.0, T0 = 1.0;
+    const real_t p0 = rho0*(u0/mach)*(u0/mach)/gamma, r_gas = p0/(rho0*T0);
+    spade::fluid_state::ideal_gas_t<real_t> air(gamma, r_gas);
+    spade::coords::identity<real_t> coords;
+    spade::ctrs::array<int, 3> num_blocks(nb, nb, nb), cells(16, 16, 16), nexch(2, 2, 2);
+    spade::bound_box_t<real_t, 3> bounds;
+    for (int d = 0; d < 3; ++d) { bounds.min(d) = 0.0; bounds.max(d) = 2.0*spade::consts::pi; }
+    spade::grid::cartesian_blocks_t blocks(num_blocks, bounds);
+    fix_boundary_flags(blocks);
+    spade::grid::cartesian_grid_t grid(cells, blocks, coords, group);
+    prim_t fill1 = 0.0;
+    flux_t fill2 = 0.0;
+    spade::grid::grid_array prim(grid, fill1, nexch, spade::device::cpu);
+    spade::grid::grid_array rhs (grid, fill2, nexch, spade::device::cpu);
+    spade::algs::fill_array(prim, [&](const spade::coords::point_t<real_t>& x) -> prim_t
+    {
+        prim_t q;
+        q.p() = p0 + (rho0*u0*u0/16.0)*(std::cos(2.0*x[0]) + std::cos(2.0*x[1]))*(std::cos(2.0*x[2]) + 2.0);
+        q.T() = T0;
+        q.u() =  u0*std::sin(x[0])*std::cos(x[1])*std::cos(x[2]);
+        q.v() = -u0*std::cos(x[0])*std::sin(x[1])*std::cos(x[2]);
+        q.w() = 0.0;
+        return q;
+    });
+    power_law_all_t<real_t> visc_pow(rho0*u0/1600.0, T0, 0.76, 0.72);
+    spade::viscous::visc_lr vsc_pow(visc_pow, air);
+    const auto flx = spade::omni::compose(spade::convective::cent_keep<2>(air), vsc_pow);
+    rhs = 0.0;
+    spade::pde_algs::flux_div(prim, rhs, flx);   // warm-up
+    std::vector<double> times;
+    for (int r = 0; r < reps; ++r)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        rhs = 0.0;
+        spade::pde_algs::flux_div(prim, rhs, flx);
+        const auto t1 = std::chrono::steady_clock::now();
+        times.push_back(std::chrono::duration<double>(t1 - t0).count());
+    }
+    std::sort(times.begin(), times.end());
+    const double med = times[times.size()/2];
+    const double cells_tot = double(nb)*nb*nb*4096.0;
+    std::printf("{\"kind\": \"reference\", \"cells\": %.0f, \"reps\": %d, \"median_s\": %.6f, \"mcell_rhs_per_s\": %.4f}\n",
+                cells_tot, reps, med, cells_tot/med*1e-6);
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
     spade::parallel::mpi_t group(&argc, &argv);
+    if (argc > 1 && std::string(argv[1]) == "--bench")
+        return run_bench(group, argc > 2 ? std::atoi(argv[2]) : 2, argc > 3 ? std::atoi(argv[3]) : 5);
     g_outdir = (argc > 1) ? argv[1] : ".";
     g_manifest.open(g_outdir + "/manifest.txt");
     g_manifest.precision(17);

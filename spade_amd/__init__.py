@@ -1,0 +1,365 @@
+"""spade_amd: MI355X-native right-hand-side hot path of SPADE.
+
+Python mirror of the reference's C++ interface for this path, calling the C
+ABI of libspade_amd.so (include/spade_amd.h):
+
+  reference (SPADE src/)                         here
+  ---------------------------------------------  ----------------------------------
+  cartesian_blocks_t + cartesian_grid_t          CartesianGrid
+  (grid/cartesian_blocks.h, cartesian_grid.h)
+  grid_array (grid/grid_array.h)                 GridArray
+  ideal_gas_t (navier-stokes/gas.h)              IdealGas
+  cent_keep<order>, totani_lr, pressure_diss_lr, cent_keep, totani_lr, pressure_diss_lr,
+  weno_t, fweno_t, first_order_t, visc_lr        weno, fweno, first_order, visc_lr
+  (navier-stokes/convective.h, viscous.h)
+  constant_viscosity_t, power_law_t              constant_viscosity, power_law
+  const_sensor_t, ducros_t                       const_sensor, ducros
+  omni::compose                                  compose
+  pde_algs::flux_div(+ overwrite/increment)      flux_div(prims, rhs, flux, increment=True)
+  create_exchange(...).exchange(arr)             create_exchange(arr).exchange(arr)
+  algs::boundary_fill (wall mirror)              boundary_fill_wall
+  integrator_t(...).advance() with ssprk3_t /    Integrator(q, flux, bc, scheme, dt).advance()
+  ssprk3_opt and state_transform_t
+
+Every array lives on the GPU; there is no CPU fallback.  If the library is
+missing, importing works but the first call raises FileNotFoundError.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi
+from ._abi import SpadeError, check, load
+
+__all__ = [
+    "Context", "CartesianGrid", "GridArray", "IdealGas", "cent_keep", "totani_lr", "pressure_diss_lr",
+    "weno", "fweno", "first_order", "visc_lr", "constant_viscosity", "power_law", "const_sensor",
+    "ducros", "compose", "flux_div", "create_exchange", "boundary_fill_wall", "Integrator",
+    "ssprk3", "ssprk3_opt", "max_wavespeed", "tanh_map", "SpadeError",
+]
+
+# ---------------------------------------------------------------- context ---
+
+
+class Context:
+    """One GPU and one HIP stream; every operation is ordered on that stream."""
+
+    def __init__(self, device=0):
+        self._h = C.c_void_p()
+        check(load().spd_ctx_create(device, C.byref(self._h)))
+        self.device = device
+
+    def sync(self):
+        check(load().spd_ctx_sync(self._h))
+
+    @property
+    def stream(self):
+        return load().spd_ctx_stream(self._h)
+
+    def timer_start(self):
+        check(load().spd_timer_start(self._h))
+
+    def timer_stop(self):
+        ms = C.c_float()
+        check(load().spd_timer_stop(self._h, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self._h:
+            load().spd_ctx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+_default_ctx = None
+
+
+def default_context():
+    global _default_ctx
+    if _default_ctx is None:
+        _default_ctx = Context(0)
+    return _default_ctx
+
+
+# ------------------------------------------------------------------- grid ---
+
+
+def tanh_map(p0, p1, beta):
+    """1-D stretching x = c + h tanh(beta s)/tanh(beta), s = (xi - c)/h on [p0, p1]."""
+    return (_abi.MAP_TANH, float(p0), float(p1), float(beta))
+
+
+class CartesianGrid:
+    """Uniform multi-block grid (cartesian_blocks_t + cartesian_grid_t).
+
+    cells_in_block, num_blocks: per-axis counts; bounds: ((x0,x1),(y0,y1),(z0,z1))
+    in computational coordinates; maps: per-axis None (identity) or tanh_map().
+    """
+
+    def __init__(self, cells_in_block, num_blocks, bounds, num_exch=2, periodic=(True, True, True),
+                 maps=(None, None, None), ctx=None):
+        self.ctx = ctx or default_context()
+        d = _abi.GridDesc()
+        for a in range(3):
+            d.num_blocks[a] = int(num_blocks[a])
+            d.cells_in_block[a] = int(cells_in_block[a])
+            d.lo[a], d.hi[a] = float(bounds[a][0]), float(bounds[a][1])
+            d.periodic[a] = int(bool(periodic[a]))
+            m = maps[a]
+            if m is None:
+                d.map[a].kind = _abi.MAP_IDENTITY
+            else:
+                d.map[a].kind, d.map[a].p0, d.map[a].p1, d.map[a].beta = m
+        d.num_exch = int(num_exch)
+        self.desc = d
+        self._h = C.c_void_p()
+        check(load().spd_grid_create(self.ctx._h, C.byref(d), C.byref(self._h)))
+        self.cells_in_block = tuple(int(c) for c in cells_in_block)
+        self.num_blocks = tuple(int(b) for b in num_blocks)
+        self.num_exch = int(num_exch)
+        self.periodic = tuple(bool(p) for p in periodic)
+
+    @property
+    def nblocks(self):
+        return self.num_blocks[0] * self.num_blocks[1] * self.num_blocks[2]
+
+    @property
+    def num_cells(self):
+        c = self.cells_in_block
+        return c[0] * c[1] * c[2] * self.nblocks
+
+    @property
+    def host_shape(self):
+        """Shape of the reference-layout host image: (blocks, Nz, Ny, Nx, 5)."""
+        ng = self.num_exch
+        c = self.cells_in_block
+        return (self.nblocks, c[2] + 2 * ng, c[1] + 2 * ng, c[0] + 2 * ng, 5)
+
+    def close(self):
+        if self._h:
+            load().spd_grid_destroy(self._h)
+            self._h = C.c_void_p()
+
+
+class GridArray:
+    """5-variable FP64 cell-centred array with ghosts (grid_array)."""
+
+    def __init__(self, grid, fill=None):
+        self.grid = grid
+        self._h = C.c_void_p()
+        check(load().spd_array_create(grid._h, C.byref(self._h)))
+        if fill is not None:
+            self.fill(fill)
+
+    def upload(self, host):
+        a = np.ascontiguousarray(host, dtype=np.float64)
+        if a.shape != self.grid.host_shape:
+            raise ValueError(f"host array shape {a.shape} != {self.grid.host_shape}")
+        check(load().spd_array_upload(self._h, a.ctypes.data_as(C.POINTER(C.c_double))))
+        return self
+
+    def download(self):
+        out = np.empty(self.grid.host_shape, dtype=np.float64)
+        check(load().spd_array_download(self._h, out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out
+
+    def fill(self, value):
+        check(load().spd_array_fill(self._h, float(value)))
+        return self
+
+    def copy_from(self, other):
+        check(load().spd_array_copy(self._h, other._h))
+        return self
+
+    def close(self):
+        if self._h:
+            load().spd_array_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- physics ---
+
+
+class IdealGas:
+    def __init__(self, gamma, R):
+        self.gamma, self.R = float(gamma), float(R)
+
+
+class _Term:
+    def __init__(self, gas, **fields):
+        self.gas = gas
+        self.fields = fields
+
+    def terms(self):
+        return [self]
+
+
+def cent_keep(order, gas):
+    return _Term(gas, kind=_abi.TERM_KEEP, order=int(order))
+
+
+def totani_lr(gas):
+    return _Term(gas, kind=_abi.TERM_TOTANI)
+
+
+class _Sensor:
+    def __init__(self, kind, param):
+        self.kind, self.param = kind, float(param)
+
+
+def const_sensor(alpha):
+    return _Sensor(_abi.SENSOR_CONST, alpha)
+
+
+def ducros(epsilon):
+    return _Sensor(_abi.SENSOR_DUCROS, epsilon)
+
+
+def pressure_diss_lr(gas, sensor, eps_p, eps_T=None):
+    return _Term(gas, kind=_abi.TERM_PDISS, sensor=sensor.kind, sensor_param=sensor.param,
+                 eps_p=float(eps_p), eps_T=float(eps_p if eps_T is None else eps_T))
+
+
+def weno(gas, split="rusanov", smooth=True):
+    return _Term(gas, kind=_abi.TERM_WENO, split=_abi.SPLIT_PHYS if split == "phys" else _abi.SPLIT_RUSANOV,
+                 smooth=int(bool(smooth)))
+
+
+def fweno(gas, smooth=True):
+    return _Term(gas, kind=_abi.TERM_FWENO, smooth=int(bool(smooth)))
+
+
+def first_order(gas):
+    return _Term(gas, kind=_abi.TERM_FIRST)
+
+
+class _Law:
+    def __init__(self, **fields):
+        self.fields = fields
+
+
+def constant_viscosity(mu, prandtl):
+    return _Law(law=_abi.LAW_CONST, mu=float(mu), prandtl=float(prandtl))
+
+
+def power_law(mu_ref, T_ref, power, prandtl):
+    return _Law(law=_abi.LAW_POWER, mu=float(mu_ref), T_ref=float(T_ref), power=float(power),
+                prandtl=float(prandtl))
+
+
+def visc_lr(law, gas):
+    return _Term(gas, kind=_abi.TERM_VISC, **law.fields)
+
+
+class _Composite:
+    def __init__(self, parts):
+        self.parts = parts
+
+    def terms(self):
+        out = []
+        for p in self.parts:
+            out.extend(p.terms())
+        return out
+
+
+def compose(*fluxes):
+    """F0 + (F1 + ...) evaluated per face (omni::compose)."""
+    return _Composite(list(fluxes))
+
+
+def flux_descriptor(flux):
+    terms = flux.terms()
+    if not 1 <= len(terms) <= _abi.MAX_TERMS:
+        raise ValueError("a flux needs 1..4 terms")
+    gas = terms[0].gas
+    d = _abi.FluxDesc()
+    d.nterms = len(terms)
+    for i, t in enumerate(terms):
+        if t.gas.gamma != gas.gamma or t.gas.R != gas.R:
+            raise ValueError("all composed fluxes must use the same gas")
+        for k, v in t.fields.items():
+            setattr(d.terms[i], k, v)
+    d.gamma, d.R = gas.gamma, gas.R
+    return d
+
+
+# ------------------------------------------------------------- algorithms ---
+
+
+def flux_div(prims, rhs, flux, increment=True):
+    """rhs (+)= -div F over interior cells (pde_algs::flux_div; default increment)."""
+    d = flux_descriptor(flux)
+    check(load().spd_flux_div(prims._h, rhs._h, C.byref(d), _abi.INCREMENT if increment else _abi.OVERWRITE))
+
+
+class ExchangeHandle:
+    def exchange(self, arr):
+        check(load().spd_exchange(arr._h))
+
+
+def create_exchange(arr, periodic=None):
+    """Ghost exchange handle; periodicity is a property of the grid here."""
+    if periodic is not None and tuple(bool(p) for p in periodic) != arr.grid.periodic:
+        raise ValueError("periodicity is fixed when the grid is created")
+    return ExchangeHandle()
+
+
+def boundary_fill_wall(arr, face, t_wall):
+    """Isothermal no-slip mirror ghosts on face = 2*dir + (0 min | 1 max)."""
+    check(load().spd_boundary_fill_wall(arr._h, int(face), float(t_wall)))
+
+
+ssprk3 = _abi.SCHEME_SSPRK3
+ssprk3_opt = _abi.SCHEME_SSPRK3_OPT
+
+
+class Integrator:
+    """integrator_t with rhs = flux_div (overwrite), bc = exchange + walls and the
+    prim<->cons state transform; the solution array q is advanced in place."""
+
+    def __init__(self, q, flux, dt, scheme=ssprk3, walls=(), t_wall=0.0):
+        self.q = q
+        self.dt = float(dt)
+        bc = _abi.BCDesc()
+        bc.wall_mask = 0
+        for face in walls:
+            bc.wall_mask |= 1 << int(face)
+        bc.t_wall = float(t_wall)
+        self._d = flux_descriptor(flux)
+        self._h = C.c_void_p()
+        check(load().spd_integrator_create(q._h, C.byref(self._d), C.byref(bc), int(scheme), self.dt,
+                                           C.byref(self._h)))
+
+    def advance(self, nsteps=1):
+        check(load().spd_integrator_advance(self._h, int(nsteps)))
+
+    def solution(self):
+        return self.q
+
+    def close(self):
+        if self._h:
+            load().spd_integrator_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def max_wavespeed(prims, gas):
+    out = C.c_double()
+    check(load().spd_max_wavespeed(prims._h, gas.gamma, gas.R, C.byref(out)))
+    return out.value

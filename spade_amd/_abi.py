@@ -1,0 +1,113 @@
+"""ctypes declarations of the libspade_amd C ABI (include/spade_amd.h).
+
+The library is loaded from this package folder only; a missing library is an
+error, never a silent fallback.
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspade_amd.so")
+
+SPD_OK, SPD_ERR_ARG, SPD_ERR_HIP, SPD_ERR_UNSUPPORTED, SPD_ERR_MEMORY = range(5)
+MAP_IDENTITY, MAP_TANH = 0, 1
+TERM_KEEP, TERM_TOTANI, TERM_PDISS, TERM_WENO, TERM_FWENO, TERM_FIRST, TERM_VISC = range(1, 8)
+SENSOR_CONST, SENSOR_DUCROS = 0, 1
+SPLIT_RUSANOV, SPLIT_PHYS = 0, 1
+LAW_CONST, LAW_POWER = 0, 1
+MAX_TERMS = 4
+OVERWRITE, INCREMENT = 0, 1
+SCHEME_SSPRK3, SCHEME_SSPRK3_OPT = 0, 1
+
+
+class Map1d(C.Structure):
+    _fields_ = [("kind", C.c_int), ("p0", C.c_double), ("p1", C.c_double), ("beta", C.c_double)]
+
+
+class GridDesc(C.Structure):
+    _fields_ = [("num_blocks", C.c_int * 3), ("cells_in_block", C.c_int * 3), ("num_exch", C.c_int),
+                ("lo", C.c_double * 3), ("hi", C.c_double * 3), ("periodic", C.c_int * 3),
+                ("map", Map1d * 3)]
+
+
+class FluxTerm(C.Structure):
+    _fields_ = [("kind", C.c_int), ("order", C.c_int), ("sensor", C.c_int),
+                ("sensor_param", C.c_double), ("eps_p", C.c_double), ("eps_T", C.c_double),
+                ("split", C.c_int), ("smooth", C.c_int), ("law", C.c_int),
+                ("mu", C.c_double), ("T_ref", C.c_double), ("power", C.c_double),
+                ("prandtl", C.c_double)]
+
+
+class FluxDesc(C.Structure):
+    _fields_ = [("nterms", C.c_int), ("terms", FluxTerm * MAX_TERMS), ("gamma", C.c_double), ("R", C.c_double)]
+
+
+class BCDesc(C.Structure):
+    _fields_ = [("wall_mask", C.c_int), ("t_wall", C.c_double)]
+
+
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+
+# name -> (restype, argtypes); every function declared in include/spade_amd.h
+SIGNATURES = {
+    "spd_last_error": (C.c_char_p, []),
+    "spd_ctx_create": (C.c_int, [C.c_int, C.POINTER(_vp)]),
+    "spd_ctx_destroy": (C.c_int, [_vp]),
+    "spd_ctx_sync": (C.c_int, [_vp]),
+    "spd_ctx_stream": (_vp, [_vp]),
+    "spd_timer_start": (C.c_int, [_vp]),
+    "spd_timer_stop": (C.c_int, [_vp, C.POINTER(C.c_float)]),
+    "spd_grid_create": (C.c_int, [_vp, C.POINTER(GridDesc), C.POINTER(_vp)]),
+    "spd_grid_destroy": (C.c_int, [_vp]),
+    "spd_array_create": (C.c_int, [_vp, C.POINTER(_vp)]),
+    "spd_array_destroy": (C.c_int, [_vp]),
+    "spd_array_upload": (C.c_int, [_vp, _dp]),
+    "spd_array_download": (C.c_int, [_vp, _dp]),
+    "spd_array_fill": (C.c_int, [_vp, C.c_double]),
+    "spd_array_copy": (C.c_int, [_vp, _vp]),
+    "spd_array_host_size": (C.c_long, [_vp]),
+    "spd_flux_div": (C.c_int, [_vp, _vp, C.POINTER(FluxDesc), C.c_int]),
+    "spd_exchange": (C.c_int, [_vp]),
+    "spd_boundary_fill_wall": (C.c_int, [_vp, C.c_int, C.c_double]),
+    "spd_integrator_create": (C.c_int, [_vp, C.POINTER(FluxDesc), C.POINTER(BCDesc), C.c_int, C.c_double,
+                                        C.POINTER(_vp)]),
+    "spd_integrator_advance": (C.c_int, [_vp, C.c_int]),
+    "spd_integrator_destroy": (C.c_int, [_vp]),
+    "spd_integrator_residual": (_vp, [_vp, C.c_int]),
+    "spd_max_wavespeed": (C.c_int, [_vp, C.c_double, C.c_double, _dp]),
+}
+
+_lib = None
+
+
+class SpadeError(RuntimeError):
+    """Raised for any non-zero status of the C ABI (reference: except::sp_exception)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def load():
+    """Load libspade_amd.so from the package folder (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise FileNotFoundError(
+            f"{LIB_PATH} is not built; run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "or `python spade_amd/build.py`")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(code):
+    if code != SPD_OK:
+        msg = load().spd_last_error().decode()
+        raise SpadeError(code, msg)

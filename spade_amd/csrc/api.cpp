@@ -1,0 +1,506 @@
+// C ABI of libspade_amd (declared in include/spade_amd.h).
+#include "spd_internal.hpp"
+#include <cmath>
+#include <cstring>
+#include <cstdlib>
+#include <memory>
+
+namespace spd {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+} // namespace spd
+
+using namespace spd;
+
+struct spd_ctx_s        { Ctx c; };
+struct spd_grid_s       { Grid g; };
+struct spd_array_s      { Array a; };
+struct spd_integrator_s
+{
+    spd_array_s* q = nullptr;
+    spd_flux_desc flux{};
+    spd_bc_desc bc{};
+    int scheme = 0;
+    double dt = 0;
+    spd_array_s* r[3] = {nullptr, nullptr, nullptr};
+};
+
+namespace {
+
+template <typename F> int guarded(F&& fn)
+{
+    try
+    {
+        fn();
+        return SPD_OK;
+    }
+    catch (const ArgError& e) { set_error(e.msg); return SPD_ERR_ARG; }
+    catch (const HipError& e)
+    {
+        set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
+        return e.code == hipErrorOutOfMemory ? SPD_ERR_MEMORY : SPD_ERR_HIP;
+    }
+    catch (const std::bad_alloc&) { set_error("host allocation failed"); return SPD_ERR_MEMORY; }
+    catch (const std::exception& e) { set_error(e.what()); return SPD_ERR_ARG; }
+}
+
+inline void require(bool ok, const std::string& msg) { if (!ok) throw ArgError{msg}; }
+
+void validate_flux(const spd_flux_desc* f, int ng)
+{
+    require(f != nullptr, "flux descriptor is null");
+    require(f->nterms >= 1 && f->nterms <= SPD_MAX_TERMS, "flux descriptor needs 1..4 terms");
+    require(f->gamma > 1.0 && f->R > 0.0, "gas needs gamma > 1 and R > 0");
+    for (int m = 0; m < f->nterms; ++m)
+    {
+        const spd_flux_term& t = f->terms[m];
+        switch (t.kind)
+        {
+            case SPD_TERM_KEEP:
+                require(t.order >= 2 && t.order % 2 == 0 && t.order <= 8, "cent_keep order must be even, 2..8");
+                require(t.order/2 <= ng, "cent_keep order needs order/2 ghost layers");
+                require(t.order <= 4, "cent_keep orders above 4 are not implemented");
+                break;
+            case SPD_TERM_WENO: case SPD_TERM_FWENO:
+                require(ng >= 2, "WENO stencils need 2 ghost layers");
+                break;
+            case SPD_TERM_TOTANI: case SPD_TERM_PDISS: case SPD_TERM_FIRST: case SPD_TERM_VISC:
+                require(ng >= 1, "flux needs at least one ghost layer");
+                break;
+            default:
+                throw ArgError{"unknown flux term kind " + std::to_string(t.kind)};
+        }
+    }
+}
+
+void host_to_device_layout(const Grid& g, const double* host, std::vector<double>& soa)
+{
+    const DevGeom& G = g.geom;
+    soa.assign((size_t)G.sb*G.nblk, 0.0);
+    const long long cells = (long long)G.N[0]*G.N[1]*G.N[2];
+    for (int lb = 0; lb < G.nblk; ++lb)
+        for (long long c = 0; c < cells; ++c)
+            for (int v = 0; v < 5; ++v)
+                soa[(size_t)(lb*G.sb + v*G.sv + c)] = host[(size_t)5*(lb*cells + c) + v];
+}
+
+void device_to_host_layout(const Grid& g, const std::vector<double>& soa, double* host)
+{
+    const DevGeom& G = g.geom;
+    const long long cells = (long long)G.N[0]*G.N[1]*G.N[2];
+    for (int lb = 0; lb < G.nblk; ++lb)
+        for (long long c = 0; c < cells; ++c)
+            for (int v = 0; v < 5; ++v)
+                host[(size_t)5*(lb*cells + c) + v] = soa[(size_t)(lb*G.sb + v*G.sv + c)];
+}
+
+void apply_bc(spd_integrator_s* it)
+{
+    Grid& g = *it->q->a.grid;
+    launch_exchange(g, it->q->a.d, g.ctx->stream);
+    for (int face = 0; face < 6; ++face)
+        if (it->bc.wall_mask & (1 << face)) launch_wall_fill(g, it->q->a.d, face, it->bc.t_wall, g.ctx->stream);
+}
+
+spd_array_s* new_array(Grid& g)
+{
+    auto* A = new spd_array_s;
+    std::unique_ptr<spd_array_s> guard(A);
+    A->a.grid = &g;
+    const size_t bytes = sizeof(double)*(size_t)g.geom.sb*g.geom.nblk;
+    SPD_HIP(hipMalloc((void**)&A->a.d, bytes));
+    SPD_HIP(hipMemsetAsync(A->a.d, 0, bytes, g.ctx->stream));
+    return guard.release();
+}
+
+void free_array(spd_array_s* A)
+{
+    if (!A) return;
+    SPD_HIP(hipStreamSynchronize(A->a.grid->ctx->stream));
+    SPD_HIP(hipFree(A->a.d));
+    delete A;
+}
+
+void run_flux_div(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f, bool increment)
+{
+    if (!launch_flux_div_fast(g, prims, rhs, f, increment, g.ctx->stream))
+        launch_flux_div_generic(g, prims, rhs, f, increment, g.ctx->stream);
+}
+
+} // namespace
+
+extern "C" {
+
+const char* spd_last_error(void) { return g_last_error.c_str(); }
+
+int spd_ctx_create(int device, spd_ctx_t* out)
+{
+    return guarded([&]
+    {
+        require(out != nullptr, "out is null");
+        int n = 0;
+        SPD_HIP(hipGetDeviceCount(&n));
+        require(device >= 0 && device < n, "device index out of range");
+        auto* c = new spd_ctx_s;
+        c->c.device = device;
+        SPD_HIP(hipSetDevice(device));
+        SPD_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
+        SPD_HIP(hipEventCreate(&c->c.ev0));
+        SPD_HIP(hipEventCreate(&c->c.ev1));
+        *out = c;
+    });
+}
+
+int spd_ctx_destroy(spd_ctx_t ctx)
+{
+    return guarded([&]
+    {
+        if (!ctx) return;
+        SPD_HIP(hipStreamSynchronize(ctx->c.stream));
+        SPD_HIP(hipEventDestroy(ctx->c.ev0));
+        SPD_HIP(hipEventDestroy(ctx->c.ev1));
+        SPD_HIP(hipStreamDestroy(ctx->c.stream));
+        delete ctx;
+    });
+}
+
+int spd_ctx_sync(spd_ctx_t ctx)
+{
+    return guarded([&] { require(ctx != nullptr, "ctx is null"); SPD_HIP(hipStreamSynchronize(ctx->c.stream)); });
+}
+
+void* spd_ctx_stream(spd_ctx_t ctx) { return ctx ? (void*)ctx->c.stream : nullptr; }
+
+int spd_timer_start(spd_ctx_t ctx)
+{
+    return guarded([&] { require(ctx != nullptr, "ctx is null"); SPD_HIP(hipEventRecord(ctx->c.ev0, ctx->c.stream)); });
+}
+
+int spd_timer_stop(spd_ctx_t ctx, float* ms)
+{
+    return guarded([&]
+    {
+        require(ctx != nullptr && ms != nullptr, "null argument");
+        SPD_HIP(hipEventRecord(ctx->c.ev1, ctx->c.stream));
+        SPD_HIP(hipEventSynchronize(ctx->c.ev1));
+        SPD_HIP(hipEventElapsedTime(ms, ctx->c.ev0, ctx->c.ev1));
+    });
+}
+
+int spd_grid_create(spd_ctx_t ctx, const spd_grid_desc* d, spd_grid_t* out)
+{
+    return guarded([&]
+    {
+        require(ctx && d && out, "null argument");
+        for (int a = 0; a < 3; ++a)
+        {
+            require(d->num_blocks[a] >= 1, "num_blocks must be >= 1");
+            require(d->cells_in_block[a] >= 1, "cells_in_block must be >= 1");
+            require(d->hi[a] > d->lo[a], "bounds must satisfy hi > lo");
+            require(d->map[a].kind == SPD_MAP_IDENTITY || d->map[a].kind == SPD_MAP_TANH, "unknown coordinate map");
+            if (d->map[a].kind == SPD_MAP_TANH)
+                require(d->map[a].p1 > d->map[a].p0 && d->map[a].beta > 0.0, "tanh map needs p1 > p0 and beta > 0");
+        }
+        require(d->num_exch >= 1 && d->num_exch <= 4, "num_exch must be in 1..4");
+        for (int a = 0; a < 3; ++a) require(d->cells_in_block[a] >= d->num_exch, "blocks must be at least num_exch cells wide");
+        auto* G = new spd_grid_s;
+        std::unique_ptr<spd_grid_s> guard(G);
+        Grid& g = G->g;
+        g.ctx = &ctx->c;
+        g.desc = *d;
+        DevGeom& dg = g.geom;
+        dg.ng = d->num_exch;
+        dg.nblk = 1;
+        for (int a = 0; a < 3; ++a)
+        {
+            dg.nc[a] = d->cells_in_block[a];
+            dg.N[a] = dg.nc[a] + 2*dg.ng;
+            dg.nb[a] = d->num_blocks[a];
+            dg.nblk *= dg.nb[a];
+            dg.map_kind[a] = d->map[a].kind;
+            dg.map_c[a] = 0.5*(d->map[a].p1 + d->map[a].p0);
+            dg.map_h[a] = 0.5*(d->map[a].p1 - d->map[a].p0);
+            dg.map_beta[a] = d->map[a].beta;
+            dg.map_tanhb[a] = d->map[a].kind == SPD_MAP_TANH ? std::tanh(d->map[a].beta) : 1.0;
+        }
+        dg.identity = (dg.map_kind[0] == SPD_MAP_IDENTITY && dg.map_kind[1] == SPD_MAP_IDENTITY && dg.map_kind[2] == SPD_MAP_IDENTITY);
+        dg.sv = (long long)dg.N[0]*dg.N[1]*dg.N[2];
+        dg.sb = ((5*dg.sv + 31)/32)*32;   // 256-byte aligned blocks
+        // geometry tables (grid/cartesian_blocks.h:38-71, cartesian_grid.h:116-145)
+        double bsize[3];
+        for (int a = 0; a < 3; ++a) bsize[a] = (d->hi[a] - d->lo[a])/d->num_blocks[a];
+        g.h_dx.resize(3*dg.nblk); g.h_inv_dx.resize(3*dg.nblk); g.h_bmin.resize(3*dg.nblk);
+        std::vector<int> nbr(27*dg.nblk, -1);
+        for (int lb = 0; lb < dg.nblk; ++lb)
+        {
+            const int b[3] = {lb % dg.nb[0], (lb / dg.nb[0]) % dg.nb[1], lb / (dg.nb[0]*dg.nb[1])};
+            for (int a = 0; a < 3; ++a)
+            {
+                const double mn = d->lo[a] + b[a]*bsize[a];
+                const double mx = mn + bsize[a];
+                g.h_bmin[3*lb + a] = mn;
+                g.h_dx[3*lb + a] = (mx - mn)/dg.nc[a];
+                g.h_inv_dx[3*lb + a] = 1.0/g.h_dx[3*lb + a];
+            }
+            // ghost region e of block lb is fed by block lb+e (wrapped); skipped
+            // when that crosses a non-periodic domain face
+            for (int ez = -1; ez <= 1; ++ez) for (int ey = -1; ey <= 1; ++ey) for (int ex = -1; ex <= 1; ++ex)
+            {
+                if (!ex && !ey && !ez) continue;
+                const int e[3] = {ex, ey, ez};
+                int s[3];
+                bool skip = false;
+                for (int a = 0; a < 3; ++a)
+                {
+                    s[a] = b[a] + e[a];
+                    if (s[a] < 0 || s[a] >= dg.nb[a])
+                    {
+                        if (!d->periodic[a]) skip = true;
+                        s[a] = (s[a] + dg.nb[a]) % dg.nb[a];
+                    }
+                }
+                if (!skip) nbr[27*lb + (ex + 1) + 3*(ey + 1) + 9*(ez + 1)] = s[0] + dg.nb[0]*(s[1] + dg.nb[1]*s[2]);
+            }
+        }
+        SPD_HIP(hipSetDevice(ctx->c.device));
+        SPD_HIP(hipMalloc((void**)&g.d_tables, sizeof(double)*9*dg.nblk));
+        SPD_HIP(hipMalloc((void**)&g.d_nbr, sizeof(int)*27*dg.nblk));
+        SPD_HIP(hipMemcpy(g.d_tables, g.h_dx.data(), sizeof(double)*3*dg.nblk, hipMemcpyHostToDevice));
+        SPD_HIP(hipMemcpy(g.d_tables + 3*dg.nblk, g.h_inv_dx.data(), sizeof(double)*3*dg.nblk, hipMemcpyHostToDevice));
+        SPD_HIP(hipMemcpy(g.d_tables + 6*dg.nblk, g.h_bmin.data(), sizeof(double)*3*dg.nblk, hipMemcpyHostToDevice));
+        SPD_HIP(hipMemcpy(g.d_nbr, nbr.data(), sizeof(int)*27*dg.nblk, hipMemcpyHostToDevice));
+        dg.dx = g.d_tables;
+        dg.inv_dx = g.d_tables + 3*dg.nblk;
+        dg.bmin = g.d_tables + 6*dg.nblk;
+        *out = guard.release();
+    });
+}
+
+int spd_grid_destroy(spd_grid_t grid)
+{
+    return guarded([&]
+    {
+        if (!grid) return;
+        SPD_HIP(hipStreamSynchronize(grid->g.ctx->stream));
+        SPD_HIP(hipFree(grid->g.d_tables));
+        SPD_HIP(hipFree(grid->g.d_nbr));
+        delete grid;
+    });
+}
+
+long spd_array_host_size(spd_grid_t grid)
+{
+    if (!grid) return 0;
+    const DevGeom& G = grid->g.geom;
+    return 5L*G.N[0]*G.N[1]*G.N[2]*G.nblk;
+}
+
+int spd_array_create(spd_grid_t grid, spd_array_t* out)
+{
+    return guarded([&]
+    {
+        require(grid && out, "null argument");
+        *out = new_array(grid->g);
+    });
+}
+
+int spd_array_destroy(spd_array_t arr)
+{
+    return guarded([&]
+    {
+        free_array(arr);
+    });
+}
+
+int spd_array_upload(spd_array_t arr, const double* host)
+{
+    return guarded([&]
+    {
+        require(arr && host, "null argument");
+        const Grid& g = *arr->a.grid;
+        std::vector<double> soa;
+        host_to_device_layout(g, host, soa);
+        SPD_HIP(hipMemcpyAsync(arr->a.d, soa.data(), soa.size()*sizeof(double), hipMemcpyHostToDevice, g.ctx->stream));
+        SPD_HIP(hipStreamSynchronize(g.ctx->stream));
+    });
+}
+
+int spd_array_download(spd_array_t arr, double* host)
+{
+    return guarded([&]
+    {
+        require(arr && host, "null argument");
+        const Grid& g = *arr->a.grid;
+        std::vector<double> soa((size_t)g.geom.sb*g.geom.nblk);
+        SPD_HIP(hipMemcpyAsync(soa.data(), arr->a.d, soa.size()*sizeof(double), hipMemcpyDeviceToHost, g.ctx->stream));
+        SPD_HIP(hipStreamSynchronize(g.ctx->stream));
+        device_to_host_layout(g, soa, host);
+    });
+}
+
+int spd_array_fill(spd_array_t arr, double value)
+{
+    return guarded([&]
+    {
+        require(arr != nullptr, "null argument");
+        launch_fill(*arr->a.grid, arr->a.d, value, arr->a.grid->ctx->stream);
+    });
+}
+
+int spd_array_copy(spd_array_t dst, spd_array_t src)
+{
+    return guarded([&]
+    {
+        require(dst && src, "null argument");
+        require(dst->a.grid == src->a.grid, "arrays live on different grids");
+        const Grid& g = *dst->a.grid;
+        SPD_HIP(hipMemcpyAsync(dst->a.d, src->a.d, sizeof(double)*(size_t)g.geom.sb*g.geom.nblk,
+                               hipMemcpyDeviceToDevice, g.ctx->stream));
+    });
+}
+
+int spd_flux_div(spd_array_t prims, spd_array_t rhs, const spd_flux_desc* flux, int mode)
+{
+    return guarded([&]
+    {
+        require(prims && rhs, "null array");
+        require(prims->a.grid == rhs->a.grid, "prims and rhs live on different grids");
+        require(prims != rhs, "prims and rhs must be different arrays");
+        require(mode == SPD_OVERWRITE || mode == SPD_INCREMENT, "mode must be SPD_OVERWRITE or SPD_INCREMENT");
+        const Grid& g = *prims->a.grid;
+        validate_flux(flux, g.geom.ng);
+        run_flux_div(g, prims->a.d, rhs->a.d, *flux, mode == SPD_INCREMENT);
+    });
+}
+
+int spd_exchange(spd_array_t arr)
+{
+    return guarded([&]
+    {
+        require(arr != nullptr, "null argument");
+        launch_exchange(*arr->a.grid, arr->a.d, arr->a.grid->ctx->stream);
+    });
+}
+
+int spd_boundary_fill_wall(spd_array_t arr, int face, double t_wall)
+{
+    return guarded([&]
+    {
+        require(arr != nullptr, "null argument");
+        require(face >= 0 && face < 6, "face must be 0..5");
+        launch_wall_fill(*arr->a.grid, arr->a.d, face, t_wall, arr->a.grid->ctx->stream);
+    });
+}
+
+int spd_integrator_create(spd_array_t q, const spd_flux_desc* flux, const spd_bc_desc* bc,
+                          int scheme, double dt, spd_integrator_t* out)
+{
+    return guarded([&]
+    {
+        require(q && out, "null argument");
+        require(scheme == SPD_SCHEME_SSPRK3 || scheme == SPD_SCHEME_SSPRK3_OPT, "unknown scheme");
+        require(dt > 0.0 && std::isfinite(dt), "dt must be positive and finite");
+        validate_flux(flux, q->a.grid->geom.ng);
+        auto* it = new spd_integrator_s;
+        std::unique_ptr<spd_integrator_s> guard(it);
+        it->q = q;
+        it->flux = *flux;
+        if (bc) it->bc = *bc;
+        if (bc)
+            for (int face = 0; face < 6; ++face)
+                if (bc->wall_mask & (1 << face))
+                    require(!q->a.grid->desc.periodic[face/2], "walls are only allowed on non-periodic axes");
+        it->scheme = scheme;
+        it->dt = dt;
+        const int nres = scheme == SPD_SCHEME_SSPRK3 ? 3 : 2;
+        try
+        {
+            for (int i = 0; i < nres; ++i) it->r[i] = new_array(*q->a.grid);
+        }
+        catch (...)
+        {
+            for (auto*& r : it->r) { if (r) { (void)hipFree(r->a.d); delete r; r = nullptr; } }
+            throw;
+        }
+        *out = guard.release();
+    });
+}
+
+int spd_integrator_advance(spd_integrator_t it, int nsteps)
+{
+    return guarded([&]
+    {
+        require(it != nullptr, "null integrator");
+        require(nsteps >= 0, "nsteps must be >= 0");
+        Grid& g = *it->q->a.grid;
+        double* q = it->q->a.d;
+        const double dt = it->dt;
+        const double gamma = it->flux.gamma, R = it->flux.R;
+        for (int n = 0; n < nsteps; ++n)
+        {
+            if (it->scheme == SPD_SCHEME_SSPRK3)
+            {
+                // rows a = {(0,0,0), (1,0,0), (1/4,1/4,0)}, b = (1/6,1/6,2/3)
+                // (time-integration/explicit.h:80-88); stage coefficients are
+                // the reduced row differences times dt (advance.h:66-95)
+                double* r[3] = {it->r[0]->a.d, it->r[1]->a.d, it->r[2]->a.d};
+                const double* rc[3] = {r[0], r[1], r[2]};
+                run_flux_div(g, q, r[0], it->flux, false);
+                { const double c[1] = {(1.0/1.0)*dt};
+                  launch_rk_transform_stage(g, q, rc, c, 1, gamma, R, g.ctx->stream); }
+                apply_bc(it);
+                run_flux_div(g, q, r[1], it->flux, false);
+                { const double c[2] = {(-3.0/4.0)*dt, (1.0/4.0)*dt};
+                  launch_rk_transform_stage(g, q, rc, c, 2, gamma, R, g.ctx->stream); }
+                apply_bc(it);
+                run_flux_div(g, q, r[2], it->flux, false);
+                { const double c[3] = {(-1.0/12.0)*dt, (-1.0/12.0)*dt, (2.0/3.0)*dt};
+                  launch_rk_transform_stage(g, q, rc, c, 3, gamma, R, g.ctx->stream); }
+                apply_bc(it);
+            }
+            else
+            {
+                double* r0 = it->r[0]->a.d; double* r1 = it->r[1]->a.d;
+                run_flux_div(g, q, r0, it->flux, false);
+                launch_rk_opt_s0(g, q, r0, dt, gamma, R, g.ctx->stream);
+                apply_bc(it);
+                run_flux_div(g, q, r1, it->flux, false);
+                launch_rk_opt_s1(g, q, r0, r1, dt, gamma, R, g.ctx->stream);
+                apply_bc(it);
+                run_flux_div(g, q, r1, it->flux, false);
+                launch_rk_opt_s2(g, q, r0, r1, dt, gamma, R, g.ctx->stream);
+                apply_bc(it);
+            }
+        }
+    });
+}
+
+int spd_integrator_destroy(spd_integrator_t it)
+{
+    return guarded([&]
+    {
+        if (!it) return;
+        for (auto* r : it->r) free_array(r);
+        delete it;
+    });
+}
+
+spd_array_t spd_integrator_residual(spd_integrator_t it, int i)
+{
+    if (!it || i < 0 || i > 2) return nullptr;
+    return it->r[i];
+}
+
+int spd_max_wavespeed(spd_array_t prims, double gamma, double R, double* out)
+{
+    return guarded([&]
+    {
+        require(prims && out, "null argument");
+        const Grid& g = *prims->a.grid;
+        *out = reduce_max_wavespeed(g, prims->a.d, gamma, R, g.ctx->stream);
+    });
+}
+
+} // extern "C"

@@ -1,0 +1,146 @@
+// Ghost exchange, wall boundary fill, array fill and reductions.
+#include "spd_internal.hpp"
+#include <cmath>
+#include <algorithm>
+
+namespace spd {
+namespace {
+
+// One thread per stored cell of a block; ghost cells copy from the block that
+// owns them (26-neighbour table built on the host, grid/cartesian_blocks.h:
+// 73-105, exchange_pattern.h:382-424).  Region r = (ex+1) + 3(ey+1) + 9(ez+1)
+// where e_d = -1 below the interior, +1 above, 0 inside.
+__global__ void __launch_bounds__(256)
+k_exchange(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a)
+{
+    const int per_blk = G.N[0]*G.N[1]*G.N[2];
+    const int lb = blockIdx.y;
+    const int c = blockIdx.x*blockDim.x + threadIdx.x;
+    if (c >= per_blk) return;
+    const int i = c % G.N[0] - G.ng;
+    const int j = (c / G.N[0]) % G.N[1] - G.ng;
+    const int k = c / (G.N[0]*G.N[1]) - G.ng;
+    const int ex = (i < 0) ? -1 : (i >= G.nc[0] ? 1 : 0);
+    const int ey = (j < 0) ? -1 : (j >= G.nc[1] ? 1 : 0);
+    const int ez = (k < 0) ? -1 : (k >= G.nc[2] ? 1 : 0);
+    if (ex == 0 && ey == 0 && ez == 0) return;
+    const int src = nbr[27*lb + (ex + 1) + 3*(ey + 1) + 9*(ez + 1)];
+    if (src < 0) return;
+    const int si = i - ex*G.nc[0], sj = j - ey*G.nc[1], sk = k - ez*G.nc[2];
+    #pragma unroll
+    for (int v = 0; v < 5; ++v) a[cell_index(G, lb, v, i, j, k)] = a[cell_index(G, src, v, si, sj, sk)];
+}
+
+// boundary_fill with the isothermal no-slip mirror (grid/boundary_fill.h:
+// 104-130): ghost (normal index < 0 or >= n, tangential over the full stored
+// range) = (p, 2 Tw - T, -u, -v, -w) of the mirror cell.
+__global__ void __launch_bounds__(256)
+k_wall_fill(DevGeom G, double* __restrict__ a, int idir, int pm, double t_wall)
+{
+    const int lb = blockIdx.y;
+    int b[3] = {lb % G.nb[0], (lb / G.nb[0]) % G.nb[1], lb / (G.nb[0]*G.nb[1])};
+    if (pm == 0 && b[idir] != 0) return;
+    if (pm == 1 && b[idir] != G.nb[idir] - 1) return;
+    int ext[3] = {G.N[0], G.N[1], G.N[2]};
+    ext[idir] = G.ng;
+    const int c = blockIdx.x*blockDim.x + threadIdx.x;
+    if (c >= ext[0]*ext[1]*ext[2]) return;
+    int idx[3] = {c % ext[0] - G.ng, (c / ext[0]) % ext[1] - G.ng, c / (ext[0]*ext[1]) - G.ng};
+    if (pm == 0) idx[idir] = -G.ng + (idx[idir] + G.ng);                 // [-ng, 0)
+    else         idx[idir] = G.nc[idir] + (idx[idir] + G.ng);            // [n, n+ng)
+    int im[3] = {idx[0], idx[1], idx[2]};
+    im[idir] = (pm == 0) ? (-1 - idx[idir]) : (2*G.nc[idir] - (idx[idir] + 1));
+    const double p = a[cell_index(G, lb, 0, im[0], im[1], im[2])];
+    const double T = a[cell_index(G, lb, 1, im[0], im[1], im[2])];
+    const double u = a[cell_index(G, lb, 2, im[0], im[1], im[2])];
+    const double v = a[cell_index(G, lb, 3, im[0], im[1], im[2])];
+    const double w = a[cell_index(G, lb, 4, im[0], im[1], im[2])];
+    a[cell_index(G, lb, 0, idx[0], idx[1], idx[2])] = p;
+    a[cell_index(G, lb, 1, idx[0], idx[1], idx[2])] = 2.0*t_wall - T;
+    a[cell_index(G, lb, 2, idx[0], idx[1], idx[2])] = -u;
+    a[cell_index(G, lb, 3, idx[0], idx[1], idx[2])] = -v;
+    a[cell_index(G, lb, 4, idx[0], idx[1], idx[2])] = -w;
+}
+
+__global__ void __launch_bounds__(256)
+k_fill(double* __restrict__ a, long long n, double value)
+{
+    const long long c = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (c < n) a[c] = value;
+}
+
+__global__ void __launch_bounds__(256)
+k_max_wavespeed(DevGeom G, const double* __restrict__ q, double gamma, double R, double* __restrict__ part)
+{
+    __shared__ double red[256];
+    const int per_blk = G.nc[0]*G.nc[1]*G.nc[2];
+    const int lb = blockIdx.y;
+    double m = 0.0;
+    for (int c = blockIdx.x*blockDim.x + threadIdx.x; c < per_blk; c += gridDim.x*blockDim.x)
+    {
+        const int i = c % G.nc[0], j = (c / G.nc[0]) % G.nc[1], k = c / (G.nc[0]*G.nc[1]);
+        const double T = q[cell_index(G, lb, 1, i, j, k)];
+        const double u = q[cell_index(G, lb, 2, i, j, k)];
+        const double v = q[cell_index(G, lb, 3, i, j, k)];
+        const double w = q[cell_index(G, lb, 4, i, j, k)];
+        const double s = sqrt(gamma*R*T) + sqrt(u*u + v*v + w*w);
+        m = fmax(m, s);
+    }
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1)
+    {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.y*gridDim.x + blockIdx.x] = red[0];
+}
+
+} // namespace
+
+void launch_exchange(const Grid& g, double* a, hipStream_t s)
+{
+    const DevGeom& G = g.geom;
+    const int per_blk = G.N[0]*G.N[1]*G.N[2];
+    hipLaunchKernelGGL(k_exchange, dim3((per_blk + 255)/256, G.nblk), dim3(256), 0, s, G, g.d_nbr, a);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_wall_fill(const Grid& g, double* a, int face, double t_wall, hipStream_t s)
+{
+    const DevGeom& G = g.geom;
+    const int idir = face/2, pm = face % 2;
+    int ext[3] = {G.N[0], G.N[1], G.N[2]};
+    ext[idir] = G.ng;
+    const int n = ext[0]*ext[1]*ext[2];
+    hipLaunchKernelGGL(k_wall_fill, dim3((n + 255)/256, G.nblk), dim3(256), 0, s, G, a, idir, pm, t_wall);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_fill(const Grid& g, double* a, double value, hipStream_t s)
+{
+    const long long n = g.geom.sb*g.geom.nblk;
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255)/256)), dim3(256), 0, s, a, n, value);
+    SPD_LAUNCH_CHECK();
+}
+
+double reduce_max_wavespeed(const Grid& g, const double* q, double gamma, double R, hipStream_t s)
+{
+    const DevGeom& G = g.geom;
+    const int per_blk = G.nc[0]*G.nc[1]*G.nc[2];
+    const int gx = std::min(8, (per_blk + 255)/256);
+    const size_t np = (size_t)gx*G.nblk;
+    double* part = nullptr;
+    SPD_HIP(hipMallocAsync((void**)&part, np*sizeof(double), s));
+    hipLaunchKernelGGL(k_max_wavespeed, dim3(gx, G.nblk), dim3(256), 0, s, G, q, gamma, R, part);
+    SPD_LAUNCH_CHECK();
+    std::vector<double> h(np);
+    SPD_HIP(hipMemcpyAsync(h.data(), part, np*sizeof(double), hipMemcpyDeviceToHost, s));
+    SPD_HIP(hipFreeAsync(part, s));
+    SPD_HIP(hipStreamSynchronize(s));
+    double m = 0.0;
+    for (double x : h) m = std::max(m, x);
+    return m;
+}
+
+} // namespace spd

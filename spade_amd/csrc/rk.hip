@@ -1,0 +1,150 @@
+// SSP-RK3 stage updates on interior cells, in conserved variables with the
+// prim<->cons transform around them (navier-stokes/fluid_state.h:102-135).
+//   * transform-aware stage (time-integration/advance.h:56-102):
+//       cons = p2c(q); cons += c_j * r_j (j in order, nonzero c_j only); q = c2p(cons)
+//   * ssprk3_opt stages s0, s1, s2 (advance.h:285-353)
+// Compiled without contraction: the combination keeps the reference's
+// rounding sequence.
+#include "spd_internal.hpp"
+
+namespace spd {
+namespace {
+
+struct CellIt
+{
+    int i, j, k, lb;
+};
+
+__device__ inline bool interior_cell(const DevGeom& G, CellIt& c)
+{
+    const long long per_blk = (long long)G.nc[0]*G.nc[1]*G.nc[2];
+    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+    if (t >= per_blk*G.nblk) return false;
+    c.lb = (int)(t / per_blk);
+    const int r = (int)(t % per_blk);
+    c.i = r % G.nc[0]; c.j = (r / G.nc[0]) % G.nc[1]; c.k = r / (G.nc[0]*G.nc[1]);
+    return true;
+}
+
+__device__ inline void p2c(double gamma, double R, const double q[5], double c[5])
+{
+    const double rho   = q[0]/(R*q[1]);
+    const double rhoU2 = rho*(q[2]*q[2] + q[3]*q[3] + q[4]*q[4]);
+    const double rhoE  = 0.5*rhoU2 + (q[0]/((gamma - 1.0)));
+    c[0] = rho; c[1] = rhoE; c[2] = rho*q[2]; c[3] = rho*q[3]; c[4] = rho*q[4];
+}
+
+__device__ inline void c2p(double gamma, double R, const double c[5], double q[5])
+{
+    const double rho = c[0];
+    const double invrho = 1.0/rho;
+    const double u = invrho*c[2], v = invrho*c[3], w = invrho*c[4];
+    const double rhoU2 = rho*(u*u + v*v + w*w);
+    const double p = (gamma - 1.0)*(c[1] - 0.5*rhoU2);
+    const double T = p/(R*rho);
+    q[0] = p; q[1] = T; q[2] = u; q[3] = v; q[4] = w;
+}
+
+struct Coefs { const double* r[3]; double c[3]; int n; };
+
+__global__ void __launch_bounds__(256)
+k_rk_transform(DevGeom G, double* __restrict__ q, Coefs C, double gamma, double R)
+{
+    CellIt ci;
+    if (!interior_cell(G, ci)) return;
+    double qq[5], cons[5];
+    long long idx[5];
+    for (int v = 0; v < 5; ++v) { idx[v] = cell_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
+    p2c(gamma, R, qq, cons);
+    for (int s = 0; s < C.n; ++s)
+    {
+        if (C.c[s] == 0.0) continue;
+        double inc[5];
+        for (int v = 0; v < 5; ++v) inc[v] = C.c[s]*C.r[s][idx[v]];
+        for (int v = 0; v < 5; ++v) cons[v] += inc[v];
+    }
+    c2p(gamma, R, cons, qq);
+    for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
+}
+
+__global__ void __launch_bounds__(256)
+k_rk_opt_s0(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, double dt, double gamma, double R)
+{
+    CellIt ci;
+    if (!interior_cell(G, ci)) return;
+    double qq[5], c[5]; long long idx[5];
+    for (int v = 0; v < 5; ++v) { idx[v] = cell_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
+    p2c(gamma, R, qq, c);
+    for (int v = 0; v < 5; ++v) c[v] += dt*r0[idx[v]];
+    c2p(gamma, R, c, qq);
+    for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
+}
+
+__global__ void __launch_bounds__(256)
+k_rk_opt_s1(DevGeom G, double* __restrict__ q, double* __restrict__ r0, const double* __restrict__ r1, double dt, double gamma, double R)
+{
+    CellIt ci;
+    if (!interior_cell(G, ci)) return;
+    double qq[5], c[5], old[5], nr[5]; long long idx[5];
+    for (int v = 0; v < 5; ++v) { idx[v] = cell_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
+    p2c(gamma, R, qq, c);
+    const double sixth_dt = (1.0/6.0)*dt;
+    for (int v = 0; v < 5; ++v) { old[v] = r0[idx[v]]; nr[v] = sixth_dt*(old[v] + r1[idx[v]]); r0[idx[v]] = nr[v]; }
+    for (int v = 0; v < 5; ++v) c[v] += (3.0/2.0)*nr[v];
+    for (int v = 0; v < 5; ++v) c[v] -= dt*old[v];
+    c2p(gamma, R, c, qq);
+    for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
+}
+
+__global__ void __launch_bounds__(256)
+k_rk_opt_s2(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, const double* __restrict__ r1, double dt, double gamma, double R)
+{
+    CellIt ci;
+    if (!interior_cell(G, ci)) return;
+    double qq[5], c[5]; long long idx[5];
+    for (int v = 0; v < 5; ++v) { idx[v] = cell_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
+    p2c(gamma, R, qq, c);
+    const double two_thirds_dt = dt*(2.0/3.0);
+    for (int v = 0; v < 5; ++v) c[v] -= (1.0/2.0)*r0[idx[v]];
+    for (int v = 0; v < 5; ++v) c[v] += two_thirds_dt*r1[idx[v]];
+    c2p(gamma, R, c, qq);
+    for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
+}
+
+inline unsigned interior_blocks(const DevGeom& G)
+{
+    const long long n = (long long)G.nc[0]*G.nc[1]*G.nc[2]*G.nblk;
+    return (unsigned)((n + 255)/256);
+}
+
+} // namespace
+
+void launch_rk_transform_stage(const Grid& g, double* q, const double* const* r, const double* coef, int nr,
+                               double gamma, double R, hipStream_t s)
+{
+    Coefs C{};
+    C.n = nr;
+    for (int j = 0; j < nr; ++j) { C.r[j] = r[j]; C.c[j] = coef[j]; }
+    hipLaunchKernelGGL(k_rk_transform, dim3(interior_blocks(g.geom)), dim3(256), 0, s, g.geom, q, C, gamma, R);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_rk_opt_s0(const Grid& g, double* q, const double* r0, double dt, double gamma, double R, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_rk_opt_s0, dim3(interior_blocks(g.geom)), dim3(256), 0, s, g.geom, q, r0, dt, gamma, R);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_rk_opt_s1(const Grid& g, double* q, double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_rk_opt_s1, dim3(interior_blocks(g.geom)), dim3(256), 0, s, g.geom, q, r0, r1, dt, gamma, R);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_rk_opt_s2(const Grid& g, double* q, const double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_rk_opt_s2, dim3(interior_blocks(g.geom)), dim3(256), 0, s, g.geom, q, r0, r1, dt, gamma, R);
+    SPD_LAUNCH_CHECK();
+}
+
+} // namespace spd

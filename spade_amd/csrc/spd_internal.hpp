@@ -1,0 +1,93 @@
+// Internal types shared by the HIP translation units of libspade_amd.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include <cstdint>
+#include "../../include/spade_amd.h"
+
+namespace spd {
+
+void set_error(const std::string& msg);
+
+struct HipError { hipError_t code; std::string where; };
+
+#define SPD_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) throw ::spd::HipError{e_, std::string(#call) + " @ " +    \
+                                                    __FILE__ + ":" + std::to_string(__LINE__)}; \
+    } while (0)
+
+#define SPD_LAUNCH_CHECK() SPD_HIP(hipGetLastError())
+
+struct ArgError { std::string msg; };
+
+// Geometry as the kernels see it.  Per-block dx / 1/dx / box minimum are
+// tabulated on the host exactly as the reference computes them
+// (grid/cartesian_grid.h:116-145) because dx = ((min+bsize)-min)/n differs in
+// the last bit between blocks.
+struct DevGeom
+{
+    int nc[3];        // cells per block
+    int ng;           // ghost layers
+    int N[3];         // stored extent nc + 2 ng
+    int nb[3];        // blocks per axis
+    int nblk;
+    long long sv;     // doubles between variables of one block (N0 N1 N2)
+    long long sb;     // doubles between blocks (5 sv, padded)
+    int identity;     // all three maps identity
+    int map_kind[3];
+    double map_c[3], map_h[3], map_beta[3], map_tanhb[3];
+    const double* dx;      // [3*lb + d]
+    const double* inv_dx;  // [3*lb + d]
+    const double* bmin;    // [3*lb + d]
+};
+
+struct Ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct Grid
+{
+    Ctx* ctx = nullptr;
+    spd_grid_desc desc{};
+    DevGeom geom{};
+    double* d_tables = nullptr;   // dx, inv_dx, bmin
+    int* d_nbr = nullptr;         // [26*lb + n] source block for ghost region n, -1 = not exchanged
+    std::vector<double> h_dx, h_inv_dx, h_bmin;
+};
+
+struct Array
+{
+    Grid* grid = nullptr;
+    double* d = nullptr;          // device data, geom.sb * nblk doubles
+};
+
+// ---- kernels' host launchers ----
+void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f,
+                             bool increment, hipStream_t s);
+bool launch_flux_div_fast(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f,
+                          bool increment, hipStream_t s);
+void launch_exchange(const Grid& g, double* a, hipStream_t s);
+void launch_wall_fill(const Grid& g, double* a, int face, double t_wall, hipStream_t s);
+void launch_fill(const Grid& g, double* a, double value, hipStream_t s);
+// RK stage: q <- c2p(p2c(q) + sum_j coef[j] * r_j) on interior cells
+void launch_rk_transform_stage(const Grid& g, double* q, const double* const* r, const double* coef, int nr,
+                               double gamma, double R, hipStream_t s);
+void launch_rk_opt_s0(const Grid& g, double* q, const double* r0, double dt, double gamma, double R, hipStream_t s);
+void launch_rk_opt_s1(const Grid& g, double* q, double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s);
+void launch_rk_opt_s2(const Grid& g, double* q, const double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s);
+double reduce_max_wavespeed(const Grid& g, const double* q, double gamma, double R, hipStream_t s);
+
+// Index of a stored cell (i, j, k in [-ng, n+ng)) of variable v in block lb.
+__host__ __device__ inline long long cell_index(const DevGeom& g, int lb, int v, int i, int j, int k)
+{
+    return (long long)lb*g.sb + (long long)v*g.sv
+         + ((long long)(k + g.ng)*g.N[1] + (j + g.ng))*g.N[0] + (i + g.ng);
+}
+
+} // namespace spd

@@ -1,0 +1,40 @@
+"""Flux configurations shared by the oracle and GPU parity tests.
+
+Each case is a list of term dicts in the oracle's / the C ABI's field names;
+the parameters are those the golden driver used (tests/golden/manifest.txt).
+"""
+import golden_io as gi
+
+_, P = gi.manifest()
+GAMMA, R = P["gamma"], P["R"]
+VC = dict(kind=7, law=0, mu=P["mu_ref"], prandtl=P["prandtl"])
+VP = dict(kind=7, law=1, mu=P["mu_ref"], T_ref=P["T_ref"], power=P["power"], prandtl=P["prandtl"])
+PDC = dict(kind=3, sensor=0, sensor_param=0.7, eps_p=0.3, eps_T=0.2)
+PDD = dict(kind=3, sensor=1, sensor_param=1e-3, eps_p=0.3, eps_T=0.2)
+K2 = dict(kind=1, order=2)
+K4 = dict(kind=1, order=4)
+
+FLUX_CASES = {
+    "keep2": [K2],
+    "keep4": [K4],
+    "totani": [dict(kind=2)],
+    "visc_const": [VC],
+    "visc_pow": [VP],
+    "keep2_visc_pow": [K2, VP],
+    "pdiss_const": [PDC],
+    "pdiss_ducros": [PDD],
+    "weno_rus": [dict(kind=4, split=0, smooth=1)],
+    "weno_rus_lin": [dict(kind=4, split=0, smooth=0)],
+    "weno_phys": [dict(kind=4, split=1, smooth=1)],
+    "fweno": [dict(kind=5, smooth=1)],
+    "fweno_lin": [dict(kind=5, smooth=0)],
+    "first_order": [dict(kind=6)],
+    "keep2_pdiss_visc_const": [K2, PDD, VC],
+}
+
+# cases whose arithmetic involves pow(): equal to the reference within
+# round-off, not bit for bit, on the GPU (device pow vs glibc pow)
+USES_POW = {"visc_pow", "keep2_visc_pow"}
+
+G1 = dict(nb=(2, 3, 2), nc=(4, 8, 4), ng=2)
+G2 = dict(nb=(2, 2, 2), nc=(8, 8, 8), ng=2)
