@@ -115,9 +115,17 @@ enum { SPD_OVERWRITE = 0, SPD_INCREMENT = 1 };   /* pde-algs/pde_traits.h:7-20 *
 
 /* rhs (+)= -div F on interior cells: pde_algs::flux_div(prims, rhs, flux, traits)
  * (pde-algs/flux-div/flux_div.h:16-56, flux_div_basic.h:17-76).  prims must have
- * current ghosts.  Overwrite mode also zeroes the ghosts of rhs, as the
- * reference's rhs = 0 does. */
+ * current ghosts.  Overwrite mode writes every interior cell of rhs; its ghost
+ * cells are left as they are (the reference's rhs = 0 also zeroes them; no
+ * kernel here reads residual ghosts). */
 int spd_flux_div(spd_array_t prims, spd_array_t rhs, const spd_flux_desc* flux, int mode);
+
+/* Which kernel spd_flux_div would run for this array and flux: *kind = 1 for
+ * the LDS-staged face-once kernel (cent_keep<2> [+pressure_diss_lr]
+ * [+visc_lr], identity coordinates, 16x16 block cross-section), 0 for the
+ * generic one-thread-per-cell kernel.  No counterpart in the reference (it has
+ * a single algorithm per trait); for tests and benchmarks. */
+int spd_flux_div_kernel(spd_array_t prims, const spd_flux_desc* flux, int* kind);
 
 /* Ghost exchange between blocks (create_exchange + exchange,
  * grid/exchange_pattern.h:304-424): periodic axes wrap, domain faces of

@@ -303,6 +303,14 @@ def flux_div(prims, rhs, flux, increment=True):
     check(load().spd_flux_div(prims._h, rhs._h, C.byref(d), _abi.INCREMENT if increment else _abi.OVERWRITE))
 
 
+def flux_div_kernel(prims, flux):
+    """'fast' (LDS face-once kernel) or 'generic' for this array and flux."""
+    d = flux_descriptor(flux)
+    kind = C.c_int()
+    check(load().spd_flux_div_kernel(prims._h, C.byref(d), C.byref(kind)))
+    return "fast" if kind.value == 1 else "generic"
+
+
 class ExchangeHandle:
     def exchange(self, arr):
         check(load().spd_exchange(arr._h))

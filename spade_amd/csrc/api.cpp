@@ -375,6 +375,16 @@ int spd_flux_div(spd_array_t prims, spd_array_t rhs, const spd_flux_desc* flux, 
     });
 }
 
+int spd_flux_div_kernel(spd_array_t prims, const spd_flux_desc* flux, int* kind)
+{
+    return guarded([&]
+    {
+        require(prims && kind, "null argument");
+        validate_flux(flux, prims->a.grid->geom.ng);
+        *kind = flux_div_fast_eligible(*prims->a.grid, *flux) ? 1 : 0;
+    });
+}
+
 int spd_exchange(spd_array_t arr)
 {
     return guarded([&]

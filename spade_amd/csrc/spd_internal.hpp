@@ -72,6 +72,7 @@ void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, co
                              bool increment, hipStream_t s);
 bool launch_flux_div_fast(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f,
                           bool increment, hipStream_t s);
+bool flux_div_fast_eligible(const Grid& g, const spd_flux_desc& f);
 void launch_exchange(const Grid& g, double* a, hipStream_t s);
 void launch_wall_fill(const Grid& g, double* a, int face, double t_wall, hipStream_t s);
 void launch_fill(const Grid& g, double* a, double value, hipStream_t s);
