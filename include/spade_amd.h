@@ -155,6 +155,57 @@ spd_array_t spd_integrator_residual(spd_integrator_t it, int i);
 /* max(|u| + sqrt(gamma R T)): the CFL wave speed (development/cuda-tgv/main.cc:155-172) */
 int spd_max_wavespeed(spd_array_t prims, double gamma, double R, double* out);
 
+/* Volume averages over interior cells of a primitive array with current ghosts:
+ * kinetic energy <rho |u|^2 / 2> (rho = p/(R T)) and enstrophy <|omega|^2 / 2>,
+ * omega from second-order central differences.  Summed per block in a fixed
+ * order, then over blocks in block order (deterministic).  No counterpart in
+ * the reference (it has no such diagnostics). */
+int spd_diagnostics(spd_array_t prims, double R, double* kinetic_energy, double* enstrophy);
+
+/* ---- batched engine: many independent periodic cases stepped together ------
+ * FP32 face fluxes about each case's reference state, FP64 conserved state,
+ * FP64 residual sum and SSP-RK3 (Shu-Osher) stage combination.  One persistent
+ * kernel per RK stage walks a work list of 16^3 blocks of all active cases; a
+ * whole step is replayed from a hipGraph.  No counterpart in the reference (it
+ * steps one grid at a time); it serves the same physics as
+ * cent_keep<2> + visc_lr on identity coordinates. */
+typedef struct spd_batch_s* spd_batch_t;
+enum { SPD_CASE_TGV = 0, SPD_CASE_VORTEX = 1 };
+typedef struct {
+    int    kind;            /* SPD_CASE_TGV: [0,2pi]^3 Taylor-Green; SPD_CASE_VORTEX: [0,10]^3 isentropic vortex */
+    int    cells[3];        /* cells per axis, positive multiples of 16 */
+    double mach, reynolds;
+    double gamma, prandtl;
+    int    law;             /* SPD_LAW_CONST or SPD_LAW_POWER (must match the batch) */
+    double power;           /* power-law exponent */
+    double cfl;             /* dt = cfl dx / max(|u| + c) of the initial state, fixed */
+} spd_case_desc;
+
+/* Pool of block slots sized by hbm_budget_bytes (3 FP64 state copies per block). */
+int spd_batch_create(spd_ctx_t ctx, long long hbm_budget_bytes, int law, spd_batch_t* out);
+int spd_batch_destroy(spd_batch_t b);
+/* Blocks a case needs, and the slots still free. */
+int spd_batch_capacity(spd_batch_t b, const spd_case_desc* d, long long* blocks_needed, long long* blocks_free);
+/* Adds a case (initial state computed on the device); fails with SPD_ERR_ARG when it does not fit. */
+int spd_batch_add_case(spd_batch_t b, const spd_case_desc* d, int* case_id);
+int spd_batch_remove_case(spd_batch_t b, int case_id);
+/* Advance every active case by nsteps SSP-RK3 steps. */
+int spd_batch_step(spd_batch_t b, int nsteps);
+int spd_batch_case_info(spd_batch_t b, int case_id, long long* cells, int* steps_done, double* dt);
+/* Conserved state (rho, rhoE, rho u, rho v, rho w), [block][var][k][j][i], 16^3 per block. */
+int spd_batch_case_download(spd_batch_t b, int case_id, double* host);
+int spd_batch_case_upload(spd_batch_t b, int case_id, const double* host);
+/* Per-step history of the volume-averaged kinetic energy and enstrophy of the
+ * state at the start of each step; *n = number of entries written (<= max). */
+int spd_batch_case_diag(spd_batch_t b, int case_id, double* ke, double* enstrophy, int max, int* n);
+/* Autotuner: times the launch shapes on the current batch (state is restored)
+ * and keeps the fastest; returns the chosen persistent grid and work order. */
+int spd_batch_tune(spd_batch_t b, int* grid, int* interleaved);
+/* 1: replay each step from a hipGraph (default); 0: launch the kernels directly. */
+int spd_batch_use_graph(spd_batch_t b, int enable);
+/* Launch shape in use (persistent workgroups, work order). */
+int spd_batch_set_shape(spd_batch_t b, int grid, int interleaved);
+
 #ifdef __cplusplus
 }
 #endif

@@ -35,7 +35,7 @@ __all__ = [
     "Context", "CartesianGrid", "GridArray", "IdealGas", "cent_keep", "totani_lr", "pressure_diss_lr",
     "weno", "fweno", "first_order", "visc_lr", "constant_viscosity", "power_law", "const_sensor",
     "ducros", "compose", "flux_div", "create_exchange", "boundary_fill_wall", "Integrator",
-    "ssprk3", "ssprk3_opt", "max_wavespeed", "tanh_map", "SpadeError",
+    "ssprk3", "ssprk3_opt", "max_wavespeed", "tanh_map", "SpadeError", "diagnostics", "CaseSpec", "Batch",
 ]
 
 # ---------------------------------------------------------------- context ---
@@ -358,6 +358,120 @@ class Integrator:
     def close(self):
         if self._h:
             load().spd_integrator_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def diagnostics(prims, gas):
+    """(volume-averaged kinetic energy, enstrophy) of a primitive array with current ghosts."""
+    ke, en = C.c_double(), C.c_double()
+    check(load().spd_diagnostics(prims._h, gas.R, C.byref(ke), C.byref(en)))
+    return ke.value, en.value
+
+
+class CaseSpec:
+    """One case of the batched engine (TGV or isentropic vortex, periodic)."""
+
+    def __init__(self, kind="tgv", cells=(64, 64, 64), mach=0.1, reynolds=1600.0, gamma=1.4, prandtl=0.72,
+                 law="power", power=0.76, cfl=0.5, steps=10):
+        self.kind, self.cells = kind, tuple(int(c) for c in cells)
+        self.mach, self.reynolds, self.gamma, self.prandtl = float(mach), float(reynolds), float(gamma), float(prandtl)
+        self.law, self.power, self.cfl, self.steps = law, float(power), float(cfl), int(steps)
+
+    @property
+    def blocks(self):
+        return (self.cells[0] // 16) * (self.cells[1] // 16) * (self.cells[2] // 16)
+
+    @property
+    def num_cells(self):
+        return self.cells[0] * self.cells[1] * self.cells[2]
+
+    def desc(self):
+        d = _abi.CaseDesc()
+        d.kind = _abi.CASE_TGV if self.kind == "tgv" else _abi.CASE_VORTEX
+        for a in range(3):
+            d.cells[a] = self.cells[a]
+        d.mach, d.reynolds, d.gamma, d.prandtl = self.mach, self.reynolds, self.gamma, self.prandtl
+        d.law = _abi.LAW_POWER if self.law == "power" else _abi.LAW_CONST
+        d.power, d.cfl = self.power, self.cfl
+        return d
+
+
+class Batch:
+    """Batched FP32-flux / FP64-state engine (device pool sized by an HBM budget)."""
+
+    BYTES_PER_BLOCK = 3 * 5 * 4096 * 8
+
+    def __init__(self, hbm_budget_bytes, law="power", ctx=None):
+        self.ctx = ctx or default_context()
+        self._h = C.c_void_p()
+        lawi = _abi.LAW_POWER if law == "power" else _abi.LAW_CONST
+        check(load().spd_batch_create(self.ctx._h, int(hbm_budget_bytes), lawi, C.byref(self._h)))
+
+    def capacity(self, spec):
+        need, free = C.c_longlong(), C.c_longlong()
+        d = spec.desc()
+        check(load().spd_batch_capacity(self._h, C.byref(d), C.byref(need), C.byref(free)))
+        return need.value, free.value
+
+    def fits(self, spec):
+        need, free = self.capacity(spec)
+        return need <= free
+
+    def add(self, spec):
+        d = spec.desc()
+        cid = C.c_int()
+        check(load().spd_batch_add_case(self._h, C.byref(d), C.byref(cid)))
+        return cid.value
+
+    def remove(self, cid):
+        check(load().spd_batch_remove_case(self._h, int(cid)))
+
+    def step(self, nsteps=1):
+        check(load().spd_batch_step(self._h, int(nsteps)))
+
+    def info(self, cid):
+        cells, steps, dt = C.c_longlong(), C.c_int(), C.c_double()
+        check(load().spd_batch_case_info(self._h, int(cid), C.byref(cells), C.byref(steps), C.byref(dt)))
+        return {"cells": cells.value, "steps_done": steps.value, "dt": dt.value}
+
+    def download(self, cid):
+        n = self.info(cid)["cells"]
+        out = np.empty(5 * n, dtype=np.float64)
+        check(load().spd_batch_case_download(self._h, int(cid), out.ctypes.data_as(C.POINTER(C.c_double))))
+        return out.reshape(n // 4096, 5, 16, 16, 16)
+
+    def upload(self, cid, cons):
+        a = np.ascontiguousarray(cons, dtype=np.float64)
+        check(load().spd_batch_case_upload(self._h, int(cid), a.ctypes.data_as(C.POINTER(C.c_double))))
+
+    def diag(self, cid):
+        steps = self.info(cid)["steps_done"]
+        ke = np.empty(max(steps, 1)); en = np.empty(max(steps, 1))
+        n = C.c_int()
+        check(load().spd_batch_case_diag(self._h, int(cid), ke.ctypes.data_as(C.POINTER(C.c_double)),
+                                         en.ctypes.data_as(C.POINTER(C.c_double)), steps, C.byref(n)))
+        return ke[:n.value].copy(), en[:n.value].copy()
+
+    def tune(self):
+        g, inter = C.c_int(), C.c_int()
+        check(load().spd_batch_tune(self._h, C.byref(g), C.byref(inter)))
+        return g.value, bool(inter.value)
+
+    def use_graph(self, enable=True):
+        check(load().spd_batch_use_graph(self._h, int(bool(enable))))
+
+    def set_shape(self, grid, interleaved=False):
+        check(load().spd_batch_set_shape(self._h, int(grid), int(bool(interleaved))))
+
+    def close(self):
+        if self._h:
+            load().spd_batch_destroy(self._h)
             self._h = C.c_void_p()
 
     def __del__(self):

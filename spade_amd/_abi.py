@@ -42,6 +42,15 @@ class FluxDesc(C.Structure):
     _fields_ = [("nterms", C.c_int), ("terms", FluxTerm * MAX_TERMS), ("gamma", C.c_double), ("R", C.c_double)]
 
 
+class CaseDesc(C.Structure):
+    _fields_ = [("kind", C.c_int), ("cells", C.c_int * 3), ("mach", C.c_double), ("reynolds", C.c_double),
+                ("gamma", C.c_double), ("prandtl", C.c_double), ("law", C.c_int), ("power", C.c_double),
+                ("cfl", C.c_double)]
+
+
+CASE_TGV, CASE_VORTEX = 0, 1
+
+
 class BCDesc(C.Structure):
     _fields_ = [("wall_mask", C.c_int), ("t_wall", C.c_double)]
 
@@ -77,6 +86,20 @@ SIGNATURES = {
     "spd_integrator_destroy": (C.c_int, [_vp]),
     "spd_integrator_residual": (_vp, [_vp, C.c_int]),
     "spd_max_wavespeed": (C.c_int, [_vp, C.c_double, C.c_double, _dp]),
+    "spd_diagnostics": (C.c_int, [_vp, C.c_double, _dp, _dp]),
+    "spd_batch_create": (C.c_int, [_vp, C.c_longlong, C.c_int, C.POINTER(_vp)]),
+    "spd_batch_destroy": (C.c_int, [_vp]),
+    "spd_batch_capacity": (C.c_int, [_vp, C.POINTER(CaseDesc), C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "spd_batch_add_case": (C.c_int, [_vp, C.POINTER(CaseDesc), C.POINTER(C.c_int)]),
+    "spd_batch_remove_case": (C.c_int, [_vp, C.c_int]),
+    "spd_batch_step": (C.c_int, [_vp, C.c_int]),
+    "spd_batch_case_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_int), _dp]),
+    "spd_batch_case_download": (C.c_int, [_vp, C.c_int, _dp]),
+    "spd_batch_case_upload": (C.c_int, [_vp, C.c_int, _dp]),
+    "spd_batch_case_diag": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_int)]),
+    "spd_batch_tune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "spd_batch_use_graph": (C.c_int, [_vp, C.c_int]),
+    "spd_batch_set_shape": (C.c_int, [_vp, C.c_int, C.c_int]),
 }
 
 _lib = None

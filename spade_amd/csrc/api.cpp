@@ -1,5 +1,6 @@
 // C ABI of libspade_amd (declared in include/spade_amd.h).
 #include "spd_internal.hpp"
+#include "batch_host.hpp"
 #include <cmath>
 #include <cstring>
 #include <cstdlib>
@@ -17,6 +18,7 @@ using namespace spd;
 struct spd_ctx_s        { Ctx c; };
 struct spd_grid_s       { Grid g; };
 struct spd_array_s      { Array a; };
+struct spd_batch_s { std::unique_ptr<Batch> b; };
 struct spd_integrator_s
 {
     spd_array_s* q = nullptr;
@@ -510,6 +512,123 @@ int spd_max_wavespeed(spd_array_t prims, double gamma, double R, double* out)
         require(prims && out, "null argument");
         const Grid& g = *prims->a.grid;
         *out = reduce_max_wavespeed(g, prims->a.d, gamma, R, g.ctx->stream);
+    });
+}
+
+int spd_diagnostics(spd_array_t prims, double R, double* ke, double* en)
+{
+    return guarded([&]
+    {
+        require(prims && ke && en, "null argument");
+        require(R > 0.0, "R must be positive");
+        const Grid& g = *prims->a.grid;
+        diagnostics(g, prims->a.d, R, *ke, *en, g.ctx->stream);
+    });
+}
+
+int spd_batch_create(spd_ctx_t ctx, long long budget, int law, spd_batch_t* out)
+{
+    return guarded([&]
+    {
+        require(ctx && out, "null argument");
+        auto* b = new spd_batch_s;
+        std::unique_ptr<spd_batch_s> guard(b);
+        b->b.reset(new Batch(&ctx->c, budget, law, 64));
+        *out = guard.release();
+    });
+}
+
+int spd_batch_destroy(spd_batch_t b)
+{
+    return guarded([&] { delete b; });
+}
+
+int spd_batch_capacity(spd_batch_t b, const spd_case_desc* d, long long* need_blocks, long long* free_blocks)
+{
+    return guarded([&]
+    {
+        require(b && d && need_blocks && free_blocks, "null argument");
+        *need_blocks = Batch::footprint(*d);
+        *free_blocks = (long long)b->b->free_slots.size();
+    });
+}
+
+int spd_batch_add_case(spd_batch_t b, const spd_case_desc* d, int* id)
+{
+    return guarded([&]
+    {
+        require(b && d && id, "null argument");
+        *id = b->b->add_case(*d);
+    });
+}
+
+int spd_batch_remove_case(spd_batch_t b, int id)
+{
+    return guarded([&] { require(b != nullptr, "null argument"); b->b->remove_case(id); });
+}
+
+int spd_batch_step(spd_batch_t b, int nsteps)
+{
+    return guarded([&] { require(b != nullptr, "null argument"); b->b->step(nsteps); });
+}
+
+int spd_batch_case_info(spd_batch_t b, int id, long long* cells, int* steps, double* dt)
+{
+    return guarded([&]
+    {
+        require(b && cells && steps && dt, "null argument");
+        const BatchCase& c = b->b->get(id);
+        *cells = 4096LL*(long long)c.slots.size();
+        *steps = c.steps_done;
+        *dt = c.dt;
+    });
+}
+
+int spd_batch_case_download(spd_batch_t b, int id, double* host)
+{
+    return guarded([&] { require(b && host, "null argument"); b->b->download(id, host); });
+}
+
+int spd_batch_case_upload(spd_batch_t b, int id, const double* host)
+{
+    return guarded([&] { require(b && host, "null argument"); b->b->upload(id, host); });
+}
+
+int spd_batch_case_diag(spd_batch_t b, int id, double* ke, double* en, int max, int* n)
+{
+    return guarded([&]
+    {
+        require(b && ke && en && n, "null argument");
+        const BatchCase& c = b->b->get(id);
+        const int m = std::min<int>(max, (int)c.ke.size());
+        for (int t = 0; t < m; ++t) { ke[t] = c.ke[t]; en[t] = c.en[t]; }
+        *n = m;
+    });
+}
+
+int spd_batch_tune(spd_batch_t b, int* grid, int* interleaved)
+{
+    return guarded([&]
+    {
+        require(b && grid && interleaved, "null argument");
+        *grid = b->b->tune();
+        *interleaved = b->b->order_interleaved ? 1 : 0;
+    });
+}
+
+int spd_batch_use_graph(spd_batch_t b, int enable)
+{
+    return guarded([&] { require(b != nullptr, "null argument"); b->b->use_graph = enable != 0; });
+}
+
+int spd_batch_set_shape(spd_batch_t b, int grid, int interleaved)
+{
+    return guarded([&]
+    {
+        require(b != nullptr, "null argument");
+        require(grid >= 1 && grid <= 65536, "grid must be in 1..65536");
+        b->b->grid = grid;
+        if (b->b->order_interleaved != (interleaved != 0)) { b->b->order_interleaved = interleaved != 0; b->b->worklist_dirty = true; }
     });
 }
 

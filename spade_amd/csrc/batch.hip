@@ -1,0 +1,421 @@
+// Batched FP32-flux / FP64-state stepping of many independent periodic cases.
+//
+// Design (the "chosen form" beside the FP64 parity path):
+//   * every case is a set of 16^3 blocks; a block's conserved state
+//     (rho, rhoE, rho u, rho v, rho w) lives in FP64, SoA, without ghost
+//     layers, in a slot of a device pool; halos are read from the neighbour
+//     blocks' interiors (per-slot 27-entry neighbour table, periodic wrap);
+//   * one persistent kernel per SSP-RK3 stage walks a work list of slots of
+//     all active cases; per block it marches over k with the plane window in
+//     LDS (as the FP64 kernel does), evaluates every face flux once in FP32,
+//     sums the residual in FP64 and applies the stage combination in FP64:
+//       u_out = alpha u0 + beta (u_in + dt L(u_in))   (Shu-Osher SSP-RK3)
+//   * FP32 only carries perturbation fluxes: the face states are split about
+//     each case's reference state (rho_ref, p_ref, T_ref, e_ref = cv T_ref);
+//     the reference parts of the KEEP fluxes reduce to
+//       d(rho)  += rho_ref D(U),   d(rhoE) += e_ref d(rho) + p_ref D(U)
+//     with D(U) = -div u evaluated per cell in FP64 from FP64 velocities, and
+//     the constant p_ref n momentum flux cancels exactly;
+//   * kinetic energy and enstrophy of the stage-1 input state are summed per
+//     block (fixed-order tree) for the per-case diagnostics history.
+#include "spd_internal.hpp"
+#include "batch.hpp"
+#include <cmath>
+#include <algorithm>
+
+namespace spd {
+namespace {
+
+constexpr int HXB = 18, HPB = 18*18;
+
+struct CellF { float p, T, u, v, w, rho, e; };   // p, T, rho, e are perturbations
+
+__device__ inline int lposb(int i, int j) { return (j + 1)*HXB + (i + 1); }
+
+template <int D> __device__ inline float velf(const CellF& c) { return D == 0 ? c.u : (D == 1 ? c.v : c.w); }
+
+// FP32 face flux of cent_keep<2> + visc_lr about the reference state.
+// Outputs (f32): continuity perturbation, energy perturbation, x/y/z momentum.
+template <int D, int LAW>
+__device__ inline void face_f32(const CaseParams& P, const CellF& L, const CellF& R, const float g[3][3], float gT,
+                                float F[5])
+{
+    const float unL = velf<D>(L), unR = velf<D>(R);
+    const float U   = 0.5f*(unL + unR);
+    const float Fcp = 0.5f*(L.rho + R.rho)*U;                // rho'-part of the mass flux
+    const float Fc  = P.f_rho_ref*U + Fcp;                    // full mass flux
+    const float ub[3] = {0.5f*(L.u + R.u), 0.5f*(L.v + R.v), 0.5f*(L.w + R.w)};
+    const float K   = L.u*R.u + L.v*R.v + L.w*R.w;
+    const float eb  = 0.5f*(L.e + R.e);
+    float Fm[3] = {Fc*ub[0], Fc*ub[1], Fc*ub[2]};
+    Fm[D] += 0.5f*(L.p + R.p);
+    float FE = 0.5f*Fc*K + Fc*eb + 0.5f*(unL*R.p + unR*L.p);
+    // viscous terms
+    const float Tf = P.f_T_ref + 0.5f*(L.T + R.T);
+    float mu;
+    if (LAW == SPD_LAW_POWER) mu = P.f_mu_ref*__powf(Tf*P.f_inv_T_law, P.f_power);
+    else                      mu = P.f_mu_ref;
+    const float visc2 = -0.6666666666666667f*mu;
+    const float cond  = mu*P.f_cp_over_pr;
+    const float div = g[0][0] + g[1][1] + g[2][2];
+    float t[3];
+    #pragma unroll
+    for (int b = 0; b < 3; ++b)
+        t[b] = (b == D) ? mu*(2.0f*g[D][D]) + visc2*div : mu*(g[D][b] + g[b][D]);
+    const float ht = ub[0]*t[0] + ub[1]*t[1] + ub[2]*t[2] + cond*gT;
+    F[0] = Fcp;
+    F[1] = FE - ht;
+    F[2] = Fm[0] - t[0];
+    F[3] = Fm[1] - t[1];
+    F[4] = Fm[2] - t[2];
+}
+
+__device__ inline float tangf(float a, float b, float c, float d, float it) { return 0.25f*it*((a - b) + (c - d)); }
+
+// slot layout of the f32 velocity ring: s[pos] = u, s[HPB + pos] = v, s[2*HPB + pos] = w
+__device__ inline void gradsf_x(const float* sm, const float* s0, const float* sp, int pL, int pR,
+                                const CellF& L, const CellF& R, float ix, float iy, float iz, float g[3][3], float& gT)
+{
+    g[0][0] = (R.u - L.u)*ix; g[0][1] = (R.v - L.v)*ix; g[0][2] = (R.w - L.w)*ix; gT = (R.T - L.T)*ix;
+    g[1][0] = tangf(s0[pL + HXB], s0[pL - HXB], s0[pR + HXB], s0[pR - HXB], iy);
+    g[1][1] = tangf(s0[HPB + pL + HXB], s0[HPB + pL - HXB], s0[HPB + pR + HXB], s0[HPB + pR - HXB], iy);
+    g[1][2] = 0.0f;
+    g[2][0] = tangf(sp[pL], sm[pL], sp[pR], sm[pR], iz);
+    g[2][1] = 0.0f;
+    g[2][2] = tangf(sp[2*HPB + pL], sm[2*HPB + pL], sp[2*HPB + pR], sm[2*HPB + pR], iz);
+}
+
+__device__ inline void gradsf_y(const float* sm, const float* s0, const float* sp, int pL, int pR,
+                                const CellF& L, const CellF& R, float ix, float iy, float iz, float g[3][3], float& gT)
+{
+    g[1][0] = (R.u - L.u)*iy; g[1][1] = (R.v - L.v)*iy; g[1][2] = (R.w - L.w)*iy; gT = (R.T - L.T)*iy;
+    g[2][0] = 0.0f;
+    g[2][1] = tangf(sp[HPB + pL], sm[HPB + pL], sp[HPB + pR], sm[HPB + pR], iz);
+    g[2][2] = tangf(sp[2*HPB + pL], sm[2*HPB + pL], sp[2*HPB + pR], sm[2*HPB + pR], iz);
+    g[0][0] = tangf(s0[pL + 1], s0[pL - 1], s0[pR + 1], s0[pR - 1], ix);
+    g[0][1] = tangf(s0[HPB + pL + 1], s0[HPB + pL - 1], s0[HPB + pR + 1], s0[HPB + pR - 1], ix);
+    g[0][2] = 0.0f;
+}
+
+__device__ inline void gradsf_z(const float* sL, const float* sR, int p, const CellF& L, const CellF& R,
+                                float ix, float iy, float iz, float g[3][3], float& gT)
+{
+    g[2][0] = (R.u - L.u)*iz; g[2][1] = (R.v - L.v)*iz; g[2][2] = (R.w - L.w)*iz; gT = (R.T - L.T)*iz;
+    g[0][0] = tangf(sL[p + 1], sL[p - 1], sR[p + 1], sR[p - 1], ix);
+    g[0][1] = 0.0f;
+    g[0][2] = tangf(sL[2*HPB + p + 1], sL[2*HPB + p - 1], sR[2*HPB + p + 1], sR[2*HPB + p - 1], ix);
+    g[1][0] = 0.0f;
+    g[1][1] = tangf(sL[HPB + p + HXB], sL[HPB + p - HXB], sR[HPB + p + HXB], sR[HPB + p - HXB], iy);
+    g[1][2] = tangf(sL[2*HPB + p + HXB], sL[2*HPB + p - HXB], sR[2*HPB + p + HXB], sR[2*HPB + p - HXB], iy);
+}
+
+struct Cons { double r, E, mu, mv, mw; };
+struct PrimD { double rho, u, v, w, p, T; };
+
+__device__ inline PrimD to_prim(const CaseParams& P, const Cons& c)
+{
+    PrimD q;
+    q.rho = c.r;
+    const double inv = 1.0/c.r;
+    q.u = c.mu*inv; q.v = c.mv*inv; q.w = c.mw*inv;
+    q.p = P.gm1*(c.E - 0.5*c.r*(q.u*q.u + q.v*q.v + q.w*q.w));
+    q.T = q.p*inv*P.inv_R;
+    return q;
+}
+
+__device__ inline CellF to_f32(const CaseParams& P, const PrimD& q)
+{
+    const double Tp = q.T - P.T_ref;
+    return CellF{(float)(q.p - P.p_ref), (float)Tp, (float)q.u, (float)q.v, (float)q.w,
+                 (float)(q.rho - P.rho_ref), (float)(P.cv*Tp)};
+}
+
+template <int LAW>
+__global__ void __launch_bounds__(256, 2)
+k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q0, double* __restrict__ Qout,
+        double alpha, double beta, int want_diag)
+{
+    __shared__ float  sU[3][3*HPB];       // u, v, w (f32) of planes k-1, k, k+1
+    __shared__ float  sS[2][4*HPB];       // p', T', rho', e' of planes k, k+1 (parity)
+    __shared__ double sV[2][3*HPB];       // u, v, w (f64) of planes k, k+1 (parity)
+    __shared__ float  sFx[5][16*17];
+    __shared__ float  sFy[5][17*16];
+    __shared__ double sRed[2][256];
+
+    const int tid = threadIdx.x;
+    const int i = tid & 15, j = tid >> 4;
+    int hi = 0, hj = 0;
+    const bool has_halo = tid < 68;
+    if (tid < 16)      { hi = -1;       hj = tid; }
+    else if (tid < 32) { hi = 16;       hj = tid - 16; }
+    else if (tid < 48) { hi = tid - 32; hj = -1; }
+    else if (tid < 64) { hi = tid - 48; hj = 16; }
+    else if (tid < 68) { const int c = tid - 64; hi = (c & 1) ? 16 : -1; hj = (c & 2) ? 16 : -1; }
+    const int pown = lposb(i, j), phalo = lposb(hi, hj);
+    const int nwork = *B.nwork;
+
+    for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+    {
+        const int slot = B.worklist[w];
+        const CaseParams& P = B.cases[B.slot_case[slot]];
+        const int* nb = B.nbr + 27*slot;
+        const float ix = (float)P.ix, iy = (float)P.iy, iz = (float)P.iz;
+
+        auto load = [&](int ci, int cj, int ck) -> Cons
+        {
+            const int e = (ci < 0 ? 0 : (ci > 15 ? 2 : 1)) + 3*(cj < 0 ? 0 : (cj > 15 ? 2 : 1))
+                        + 9*(ck < 0 ? 0 : (ck > 15 ? 2 : 1));
+            const long long base = (long long)nb[e]*5*4096 + (((ck & 15)*16 + (cj & 15))*16 + (ci & 15));
+            return Cons{Qin[base], Qin[base + 4096], Qin[base + 2*4096], Qin[base + 3*4096], Qin[base + 4*4096]};
+        };
+        auto slotk = [](int k) { return (k + 3) % 3; };
+        auto put = [&](int k, int p, const PrimD& q, const CellF& c)
+        {
+            float* s = sU[slotk(k)];
+            s[p] = c.u; s[HPB + p] = c.v; s[2*HPB + p] = c.w;
+            float* t = sS[k & 1];
+            t[p] = c.p; t[HPB + p] = c.T; t[2*HPB + p] = c.rho; t[3*HPB + p] = c.e;
+            double* d = sV[k & 1];
+            d[p] = q.u; d[HPB + p] = q.v; d[2*HPB + p] = q.w;
+        };
+        auto from_lds = [&](const float* t, const float* s0, int p) -> CellF
+        {
+            return CellF{t[p], t[HPB + p], s0[p], s0[HPB + p], s0[2*HPB + p], t[2*HPB + p], t[3*HPB + p]};
+        };
+
+        // prologue: planes -1, 0, 1 resident, plane 2 in flight
+        const PrimD qprev = to_prim(P, load(i, j, -1));
+        PrimD qcur = to_prim(P, load(i, j, 0));
+        PrimD qnxt = to_prim(P, load(i, j, 1));
+        CellF prev = to_f32(P, qprev), cur = to_f32(P, qcur), nxt = to_f32(P, qnxt);
+        Cons nraw = load(i, j, 2), hraw{};
+        {
+            float* s = sU[slotk(-1)];
+            s[pown] = prev.u; s[HPB + pown] = prev.v; s[2*HPB + pown] = prev.w;
+        }
+        put(0, pown, qcur, cur); put(1, pown, qnxt, nxt);
+        if (has_halo)
+        {
+            const CellF hp = to_f32(P, to_prim(P, load(hi, hj, -1)));
+            float* s = sU[slotk(-1)];
+            s[phalo] = hp.u; s[HPB + phalo] = hp.v; s[2*HPB + phalo] = hp.w;
+            const PrimD h0 = to_prim(P, load(hi, hj, 0)), h1 = to_prim(P, load(hi, hj, 1));
+            put(0, phalo, h0, to_f32(P, h0));
+            put(1, phalo, h1, to_f32(P, h1));
+            hraw = load(hi, hj, 2);
+        }
+        double ke_acc = 0.0, en_acc = 0.0;
+        double wprev_u = qprev.u, wprev_v = qprev.v, wprev_w = qprev.w;   // own column, plane k-1 (f64)
+        __syncthreads();
+
+        float zlo[5];
+        {
+            float g[3][3], gT;
+            gradsf_z(sU[slotk(-1)], sU[slotk(0)], pown, prev, cur, ix, iy, iz, g, gT);
+            face_f32<2, LAW>(P, prev, cur, g, gT, zlo);
+        }
+
+        for (int k = 0; k < 16; ++k)
+        {
+            const float* sm = sU[slotk(k - 1)];
+            const float* s0 = sU[slotk(k)];
+            const float* sp = sU[slotk(k + 1)];
+            const float* sk = sS[k & 1];
+            {
+                const int pL = pown - 1;
+                const CellF L = from_lds(sk, s0, pL);
+                float g[3][3], gT, F[5];
+                gradsf_x(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
+                face_f32<0, LAW>(P, L, cur, g, gT, F);
+                for (int n = 0; n < 5; ++n) sFx[n][j*17 + i] = F[n];
+            }
+            {
+                const int pL = pown - HXB;
+                const CellF L = from_lds(sk, s0, pL);
+                float g[3][3], gT, F[5];
+                gradsf_y(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
+                face_f32<1, LAW>(P, L, cur, g, gT, F);
+                for (int n = 0; n < 5; ++n) sFy[n][j*16 + i] = F[n];
+            }
+            if (tid >= 128 && tid < 144)
+            {
+                const int jj = tid - 128;
+                const int pL = lposb(15, jj), pR = lposb(16, jj);
+                const CellF L = from_lds(sk, s0, pL), Rr = from_lds(sk, s0, pR);
+                float g[3][3], gT, F[5];
+                gradsf_x(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
+                face_f32<0, LAW>(P, L, Rr, g, gT, F);
+                for (int n = 0; n < 5; ++n) sFx[n][jj*17 + 16] = F[n];
+            }
+            if (tid >= 192 && tid < 208)
+            {
+                const int ii = tid - 192;
+                const int pL = lposb(ii, 15), pR = lposb(ii, 16);
+                const CellF L = from_lds(sk, s0, pL), Rr = from_lds(sk, s0, pR);
+                float g[3][3], gT, F[5];
+                gradsf_y(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
+                face_f32<1, LAW>(P, L, Rr, g, gT, F);
+                for (int n = 0; n < 5; ++n) sFy[n][16*16 + ii] = F[n];
+            }
+            float zhi[5];
+            {
+                float g[3][3], gT;
+                gradsf_z(s0, sp, pown, cur, nxt, ix, iy, iz, g, gT);
+                face_f32<2, LAW>(P, cur, nxt, g, gT, zhi);
+            }
+            // FP64 velocity differences of the cell (planes k-1..k+1)
+            const double* vk = sV[k & 1];
+            const double dux = vk[pown + 1] - vk[pown - 1];                       // u(i+1) - u(i-1)
+            const double dvy = vk[HPB + pown + HXB] - vk[HPB + pown - HXB];        // v(j+1) - v(j-1)
+            const double dwz = qnxt.w - wprev_w;                                   // w(k+1) - w(k-1)
+            const double DU = -0.5*(dux*P.ix + dvy*P.iy + dwz*P.iz);               // -div u
+            if (want_diag)
+            {
+                const double dwy = vk[2*HPB + pown + HXB] - vk[2*HPB + pown - HXB];
+                const double duy = vk[pown + HXB] - vk[pown - HXB];
+                const double dvx = vk[HPB + pown + 1] - vk[HPB + pown - 1];
+                const double dwx = vk[2*HPB + pown + 1] - vk[2*HPB + pown - 1];
+                const double dvz = qnxt.v - wprev_v, duz = qnxt.u - wprev_u;
+                const double ox = 0.5*(dwy*P.iy - dvz*P.iz);
+                const double oy = 0.5*(duz*P.iz - dwx*P.ix);
+                const double oz = 0.5*(dvx*P.ix - duy*P.iy);
+                ke_acc += 0.5*qcur.rho*(qcur.u*qcur.u + qcur.v*qcur.v + qcur.w*qcur.w);
+                en_acc += 0.5*(ox*ox + oy*oy + oz*oz);
+            }
+            __syncthreads();
+
+            // FP64 residual sum and stage combination for cell (i,j,k)
+            const double fi[3] = {P.ix, P.iy, P.iz};
+            double r[5];
+            #pragma unroll
+            for (int n = 0; n < 5; ++n)
+            {
+                r[n] = (double)sFx[n][j*17 + i]*fi[0] - (double)sFx[n][j*17 + i + 1]*fi[0]
+                     + (double)sFy[n][j*16 + i]*fi[1] - (double)sFy[n][(j + 1)*16 + i]*fi[1]
+                     + (double)zlo[n]*fi[2] - (double)zhi[n]*fi[2];
+                zlo[n] = zhi[n];
+            }
+            const double d_rho = r[0] + P.rho_ref*DU;
+            const double d_E   = r[1] + P.e_ref*d_rho + P.p_ref*DU;
+            const double res[5] = {d_rho, d_E, r[2], r[3], r[4]};
+            const long long o = (long long)slot*5*4096 + (k*16 + j)*16 + i;
+            #pragma unroll
+            for (int n = 0; n < 5; ++n)
+            {
+                const double uin = Qin[o + n*4096];
+                const double stage = uin + P.dt*res[n];
+                Qout[o + n*4096] = (alpha == 0.0) ? beta*stage : alpha*Q0[o + n*4096] + beta*stage;
+            }
+
+            // advance the window
+            wprev_u = qcur.u; wprev_v = qcur.v; wprev_w = qcur.w;
+            if (k + 1 < 16)
+            {
+                qcur = qnxt; cur = nxt;
+                qnxt = to_prim(P, nraw);
+                nxt = to_f32(P, qnxt);
+                put(k + 2, pown, qnxt, nxt);
+                if (k + 3 <= 16) nraw = load(i, j, k + 3);
+                if (has_halo)
+                {
+                    const PrimD hq = to_prim(P, hraw);
+                    put(k + 2, phalo, hq, to_f32(P, hq));
+                    if (k + 3 <= 16) hraw = load(hi, hj, k + 3);
+                }
+            }
+            __syncthreads();
+        }
+        if (want_diag)
+        {
+            sRed[0][tid] = ke_acc; sRed[1][tid] = en_acc;
+            __syncthreads();
+            for (int s = 128; s > 0; s >>= 1)
+            {
+                if (tid < s) { sRed[0][tid] += sRed[0][tid + s]; sRed[1][tid] += sRed[1][tid + s]; }
+                __syncthreads();
+            }
+            if (tid == 0)
+            {
+                const int h = *B.step % B.hist_len;
+                B.diag[((long long)h*B.max_slots + slot)*2 + 0] = sRed[0][0];
+                B.diag[((long long)h*B.max_slots + slot)*2 + 1] = sRed[1][0];
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ void k_step_count(int* step) { if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1; }
+
+// Initial conditions for the blocks of one case (TGV or isentropic vortex), in
+// conserved FP64 variables.
+__global__ void __launch_bounds__(256)
+k_init_case(BatchDev B, const int* __restrict__ slots, int nslots, CaseParams P, CaseIC ic, double* __restrict__ Q)
+{
+    const int b = blockIdx.y;
+    if (b >= nslots) return;
+    const int slot = slots[b];
+    const int bx = b % ic.nb[0], by = (b / ic.nb[0]) % ic.nb[1], bz = b / (ic.nb[0]*ic.nb[1]);
+    for (int c = threadIdx.x + blockIdx.x*blockDim.x; c < 4096; c += blockDim.x*gridDim.x)
+    {
+        const int i = c & 15, j = (c >> 4) & 15, k = c >> 8;
+        const double x = (bx*16 + i + 0.5)*ic.dx[0];
+        const double y = (by*16 + j + 0.5)*ic.dx[1];
+        const double z = (bz*16 + k + 0.5)*ic.dx[2];
+        double p, T, u, v, w;
+        if (ic.kind == SPD_CASE_TGV)
+        {
+            p = ic.p0 + (ic.rho0*ic.u0*ic.u0/16.0)*(cos(2.0*x) + cos(2.0*y))*(cos(2.0*z) + 2.0);
+            T = ic.T0;
+            u =  ic.u0*sin(x)*cos(y)*cos(z);
+            v = -ic.u0*cos(x)*sin(y)*cos(z);
+            w = 0.0;
+        }
+        else
+        {
+            const double xr = x - ic.xc, yr = y - ic.yc;
+            const double r2 = xr*xr + yr*yr;
+            const double f = exp(0.5*(1.0 - r2));
+            u = ic.u0 - ic.beta/(2.0*M_PI)*yr*f;
+            v = ic.beta/(2.0*M_PI)*xr*f;
+            w = 0.0;
+            T = ic.T0 - ic.dT*exp(1.0 - r2);
+            const double rho = ic.rho0*pow(T/ic.T0, 1.0/(P.gamma - 1.0));
+            p = rho*P.R*T;
+        }
+        const double rho = p/(P.R*T);
+        const long long o = (long long)slot*5*4096 + c;
+        Q[o]          = rho;
+        Q[o + 4096]   = 0.5*rho*(u*u + v*v + w*w) + p/(P.gamma - 1.0);
+        Q[o + 2*4096] = rho*u;
+        Q[o + 3*4096] = rho*v;
+        Q[o + 4*4096] = rho*w;
+    }
+}
+
+} // namespace
+
+void launch_batch_stage(const BatchDev& B, int law, const double* Qin, const double* Q0, double* Qout,
+                        double alpha, double beta, int want_diag, int grid, hipStream_t s)
+{
+    if (law == SPD_LAW_POWER)
+        hipLaunchKernelGGL(k_stage<SPD_LAW_POWER>, dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag);
+    else
+        hipLaunchKernelGGL(k_stage<SPD_LAW_CONST>, dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_batch_step_count(const BatchDev& B, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_count, dim3(1), dim3(64), 0, s, B.step);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_batch_init(const BatchDev& B, const int* d_slots, int nslots, const CaseParams& P, const CaseIC& ic,
+                       double* Q, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_init_case, dim3(4, nslots), dim3(256), 0, s, B, d_slots, nslots, P, ic, Q);
+    SPD_LAUNCH_CHECK();
+}
+
+} // namespace spd

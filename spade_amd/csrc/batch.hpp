@@ -1,0 +1,48 @@
+// Device-side tables of the batched engine (see batch.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace spd {
+
+// Per-case constants; FP64 for the state arithmetic, f_* copies for the FP32 fluxes.
+struct CaseParams
+{
+    double dt;
+    double ix, iy, iz;                       // 1 / dx per direction
+    double gamma, gm1, R, inv_R, cv;         // ideal gas
+    double rho_ref, p_ref, T_ref, e_ref;     // reference state the FP32 fluxes are split about
+    float  f_rho_ref, f_T_ref;
+    float  f_mu_ref, f_inv_T_law, f_power, f_cp_over_pr;
+    int    law;
+};
+
+// Initial condition of a case (TGV or isentropic vortex)
+struct CaseIC
+{
+    int    kind;                             // SPD_CASE_*
+    int    nb[3];                            // blocks per axis
+    double dx[3];
+    double rho0, u0, p0, T0;
+    double xc, yc, beta, dT;                 // vortex centre, strength, temperature dip
+};
+
+struct BatchDev
+{
+    const int* worklist;                     // slots to advance
+    const int* nwork;                        // number of entries in worklist (device-side)
+    const int* slot_case;                    // slot -> case table index
+    const int* nbr;                          // [27*slot + e] neighbour slot (periodic within case)
+    const CaseParams* cases;
+    double* diag;                            // [hist][slot][2]: kinetic energy, enstrophy sums
+    int* step;                               // device step counter (history row)
+    int max_slots;
+    int hist_len;
+};
+
+void launch_batch_stage(const BatchDev& B, int law, const double* Qin, const double* Q0, double* Qout,
+                        double alpha, double beta, int want_diag, int grid, hipStream_t s);
+void launch_batch_step_count(const BatchDev& B, hipStream_t s);
+void launch_batch_init(const BatchDev& B, const int* d_slots, int nslots, const CaseParams& P, const CaseIC& ic,
+                       double* Q, hipStream_t s);
+
+} // namespace spd

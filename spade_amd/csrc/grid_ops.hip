@@ -96,7 +96,56 @@ k_max_wavespeed(DevGeom G, const double* __restrict__ q, double gamma, double R,
     if (threadIdx.x == 0) part[blockIdx.y*gridDim.x + blockIdx.x] = red[0];
 }
 
+// Per-block sums of rho |u|^2 / 2 and |omega|^2 / 2 (omega from central
+// differences; needs current ghosts).  One workgroup per block, fixed-order tree.
+__global__ void __launch_bounds__(256)
+k_diag(DevGeom G, const double* __restrict__ q, double R, double* __restrict__ part)
+{
+    __shared__ double red[2][256];
+    const int lb = blockIdx.x;
+    const int per_blk = G.nc[0]*G.nc[1]*G.nc[2];
+    const double ix = G.inv_dx[3*lb], iy = G.inv_dx[3*lb + 1], iz = G.inv_dx[3*lb + 2];
+    double ke = 0.0, en = 0.0;
+    for (int c = threadIdx.x; c < per_blk; c += blockDim.x)
+    {
+        const int i = c % G.nc[0], j = (c / G.nc[0]) % G.nc[1], k = c / (G.nc[0]*G.nc[1]);
+        auto Q = [&](int v, int a, int b, int d) { return q[cell_index(G, lb, v, a, b, d)]; };
+        const double p = Q(0, i, j, k), T = Q(1, i, j, k);
+        const double u = Q(2, i, j, k), v = Q(3, i, j, k), w = Q(4, i, j, k);
+        ke += 0.5*(p/(R*T))*(u*u + v*v + w*w);
+        const double ox = 0.5*((Q(4, i, j + 1, k) - Q(4, i, j - 1, k))*iy - (Q(3, i, j, k + 1) - Q(3, i, j, k - 1))*iz);
+        const double oy = 0.5*((Q(2, i, j, k + 1) - Q(2, i, j, k - 1))*iz - (Q(4, i + 1, j, k) - Q(4, i - 1, j, k))*ix);
+        const double oz = 0.5*((Q(3, i + 1, j, k) - Q(3, i - 1, j, k))*ix - (Q(2, i, j + 1, k) - Q(2, i, j - 1, k))*iy);
+        en += 0.5*(ox*ox + oy*oy + oz*oz);
+    }
+    red[0][threadIdx.x] = ke; red[1][threadIdx.x] = en;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s) { red[0][threadIdx.x] += red[0][threadIdx.x + s]; red[1][threadIdx.x] += red[1][threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[2*lb] = red[0][0]; part[2*lb + 1] = red[1][0]; }
+}
+
 } // namespace
+
+void diagnostics(const Grid& g, const double* q, double R, double& ke, double& en, hipStream_t s)
+{
+    const DevGeom& G = g.geom;
+    double* part = nullptr;
+    SPD_HIP(hipMallocAsync((void**)&part, sizeof(double)*2*G.nblk, s));
+    hipLaunchKernelGGL(k_diag, dim3(G.nblk), dim3(256), 0, s, G, q, R, part);
+    SPD_LAUNCH_CHECK();
+    std::vector<double> h(2*(size_t)G.nblk);
+    SPD_HIP(hipMemcpyAsync(h.data(), part, sizeof(double)*2*G.nblk, hipMemcpyDeviceToHost, s));
+    SPD_HIP(hipFreeAsync(part, s));
+    SPD_HIP(hipStreamSynchronize(s));
+    double a = 0.0, b = 0.0;
+    for (int lb = 0; lb < G.nblk; ++lb) { a += h[2*lb]; b += h[2*lb + 1]; }
+    const double n = (double)G.nc[0]*G.nc[1]*G.nc[2]*G.nblk;
+    ke = a/n; en = b/n;
+}
 
 void launch_exchange(const Grid& g, double* a, hipStream_t s)
 {

@@ -82,6 +82,7 @@ void launch_rk_transform_stage(const Grid& g, double* q, const double* const* r,
 void launch_rk_opt_s0(const Grid& g, double* q, const double* r0, double dt, double gamma, double R, hipStream_t s);
 void launch_rk_opt_s1(const Grid& g, double* q, double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s);
 void launch_rk_opt_s2(const Grid& g, double* q, const double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s);
+void diagnostics(const Grid& g, const double* q, double R, double& ke, double& en, hipStream_t s);
 double reduce_max_wavespeed(const Grid& g, const double* q, double gamma, double R, hipStream_t s);
 
 // Index of a stored cell (i, j, k in [-ng, n+ng)) of variable v in block lb.

@@ -1,0 +1,130 @@
+"""Continuous-batching request loop over the batched engine.
+
+Requests (one case each: TGV or isentropic vortex, with its own size, Mach,
+Reynolds number and step count) arrive over time.  The loop
+
+  * admits queued requests in arrival order while their HBM footprint (16^3
+    blocks x three FP64 state copies) fits in the engine's pool, letting a
+    later, smaller request pass a head request that does not fit yet
+    (backfill), so the device stays busy;
+  * keeps every admitted case resident on the device and advances all of them
+    together, a few SSP-RK3 steps per round, so newly admitted cases join the
+    running ones and finished cases leave between rounds;
+  * retires a case when it has done its step count: its final conserved
+    fields and its per-step kinetic-energy / enstrophy history are copied back
+    and its slots are freed for the queue.
+
+Per-case arithmetic does not depend on which other cases share the device
+(the work list only changes which blocks a kernel visits, never how a block
+is computed; per-case reductions run in the case's own block order), so a
+case's result is bit-for-bit the same alone and in any mix.
+
+The engine is anything with fits(spec), add(spec) -> id, step(k), download(id),
+diag(id), remove(id); spade_amd.Batch is the device one.
+"""
+import collections
+import time
+
+import numpy as np
+
+
+class Request:
+    def __init__(self, rid, spec, arrival):
+        self.rid, self.spec, self.arrival = rid, spec, float(arrival)
+
+
+class Result:
+    def __init__(self, request, admitted, completed, state, ke, enstrophy):
+        self.request = request
+        self.admitted, self.completed = admitted, completed
+        self.state, self.ke, self.enstrophy = state, ke, enstrophy
+
+    @property
+    def latency(self):
+        return self.completed - self.request.arrival
+
+
+def poisson_requests(n, rate, make_spec, seed=0):
+    """n requests with exponential inter-arrival times of mean 1/rate seconds."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    for rid in range(n):
+        t += rng.exponential(1.0 / rate) if rate > 0 else 0.0
+        out.append(Request(rid, make_spec(rng), t))
+    return out
+
+
+class Server:
+    def __init__(self, engine, max_chunk=8, backfill=True, keep_state=True, clock=time.perf_counter,
+                 sleep=time.sleep):
+        self.engine = engine
+        self.max_chunk = int(max_chunk)
+        self.backfill = backfill
+        self.keep_state = keep_state
+        self.clock, self.sleep = clock, sleep
+        self.rounds = 0
+        self.cell_updates = 0
+
+    def run(self, requests):
+        """Serve requests (arrival times relative to the start); returns results in completion order."""
+        pending = collections.deque(sorted(requests, key=lambda r: r.arrival))
+        queue = collections.deque()
+        active = {}                      # case id -> [request, remaining steps, admitted time]
+        results = []
+        t0 = self.clock()
+        self.start_time = t0
+        while pending or queue or active:
+            now = self.clock() - t0
+            while pending and pending[0].arrival <= now:
+                queue.append(pending.popleft())
+            self._admit(queue, active, now)
+            if not active:
+                if pending:
+                    wait = pending[0].arrival - (self.clock() - t0)
+                    if wait > 0:
+                        self.sleep(min(wait, 0.05))
+                    continue
+                if queue:
+                    raise RuntimeError(f"request {queue[0].rid} can never fit in the engine's pool")
+                break
+            k = min(self.max_chunk, min(a[1] for a in active.values()))
+            self.engine.step(k)
+            self.rounds += 1
+            done_at = self.clock() - t0
+            for cid in list(active):
+                a = active[cid]
+                a[1] -= k
+                self.cell_updates += k * a[0].spec.num_cells
+                if a[1] == 0:
+                    state = self.engine.download(cid) if self.keep_state else None
+                    ke, en = self.engine.diag(cid)
+                    self.engine.remove(cid)
+                    results.append(Result(a[0], a[2], done_at, state, ke, en))
+                    del active[cid]
+        self.makespan = self.clock() - t0
+        return results
+
+    def _admit(self, queue, active, now):
+        i = 0
+        while i < len(queue):
+            req = queue[i]
+            if self.engine.fits(req.spec):
+                cid = self.engine.add(req.spec)
+                active[cid] = [req, req.spec.steps, now]
+                del queue[i]
+                continue
+            if not self.backfill:
+                break
+            i += 1
+
+
+def summarize(results, makespan, cell_updates):
+    lat = np.array([r.latency for r in results])
+    return {
+        "requests": len(results),
+        "p50_latency_s": float(np.percentile(lat, 50)) if len(lat) else None,
+        "p99_latency_s": float(np.percentile(lat, 99)) if len(lat) else None,
+        "makespan_s": makespan,
+        "cell_updates_per_s": cell_updates / makespan if makespan > 0 else None,
+    }
