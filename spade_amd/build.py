@@ -52,7 +52,21 @@ def build(verbose=True, jobs=None):
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(tmp, LIB)
+    build_facade_demo(verbose)
     return LIB
+
+
+def build_facade_demo(verbose=True):
+    """The C++ facade example (tests/cpp/facade_demo.cpp), linked against the library."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "facade_demo.cpp")
+    exe = os.path.join(OBJDIR, "facade_demo")
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-I", os.path.join(root, "include"), src,
+           "-o", exe, "-L", HERE, "-lspade_amd", "-Wl,-rpath,$ORIGIN/.."]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    return exe
 
 
 def clean():
