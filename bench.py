@@ -118,6 +118,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--blocks", type=int, default=16, help="blocks per axis (16 -> 256^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-check", type=int, default=0,
+                    help="after the timed loop, copy the primitive array N times (a plain device copy of "
+                         "known size, used to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by blocks "
@@ -175,6 +178,13 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
+    if args.copy_check:
+        for _ in range(args.copy_check):
+            rhs.copy_from(q)
+        g.ctx.sync()
+        # the device array of this geometry is the host image without padding:
+        # 20^3 doubles per variable is a multiple of the 256-byte variable stride
+        line["copy_check_bytes"] = 8 * int(np.prod(g.host_shape))
     if not args.no_cpu_baseline:
         cb = cpu_baseline()
         line["cpu_baseline"] = cb

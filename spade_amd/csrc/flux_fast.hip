@@ -18,12 +18,13 @@
 // (navier-stokes/convective.h:117-184, 212-245; viscous.h:39-112;
 // omni/infos/info_gradient.h:45-81; pde-algs/flux-div/flux_div_basic.h:47-72)
 // and the file is compiled without contraction, so the result equals the
-// reference bit for bit except through pow() of the power-law viscosity.
+// reference bit for bit except through pow() of the power-law viscosity
+// (a binomial series within one ulp of x^n, see pow_series.hpp).
 // Per-cell quantities the reference recomputes for every face (rho = p/(R T),
 // e = p/(rho (gamma-1))) are computed once per cell: same operands, same
 // rounding.
 #include "spd_internal.hpp"
-#include "dd_pow.hpp"
+#include "pow_series.hpp"
 
 namespace spd {
 namespace {
@@ -39,6 +40,7 @@ struct FastCfg
     double mu_s, visc2_s, cond_s;   // constant law: s mu, s beta, s cp alpha
     double mu_ref_s, T_ref, inv_T_ref, power, prandtl, inv_prandtl;
     int    t_ref_one;               // T_ref == 1: T/T_ref == T exactly
+    PowSeries ps;                   // (T/T_ref)^power near T = T_ref
     // pressure diffusion, sensor pre-multiplied by s
     double scl_s, ducros_eps, ducros_s, eps_p, eps_T;
 };
@@ -67,11 +69,11 @@ __device__ inline double div_const(double a, double b, double y)
 // library pow, kept out of line: it only runs for T/T_ref far from 1
 __device__ __attribute__((noinline)) double pow_far(double x, double n) { return pow(x, n); }
 
-// x^n of the power law: double-double evaluation near 1, library pow otherwise
-__device__ inline double pow_law(double x, double n)
+// x^n of the power law: binomial series near 1, library pow otherwise
+__device__ inline double pow_law(const FastCfg& C, double x)
 {
-    if (PowNearOne::in_range(x, n)) return dd_pow_near_one(x, n);
-    return pow_far(x, n);
+    if (C.ps.in_range(x)) return C.ps.eval(x);
+    return pow_far(x, C.power);
 }
 
 // ducros_t::get_sensor (state_sensor.h:32-42); g[c][q], q = u, v, w
@@ -133,7 +135,7 @@ __device__ inline void visc_s(const FastCfg& C, const FaceSums& f, const double 
     {
         const double Tf = f.ST*0.5;
         const double x = C.t_ref_one ? Tf : div_const(Tf, C.T_ref, C.inv_T_ref);
-        mu    = C.mu_ref_s*pow_law(x, C.power);
+        mu    = C.mu_ref_s*pow_law(C, x);
         visc2 = -0.66666666667*mu;
         cond  = C.spec_heat*div_const(mu, C.prandtl, C.inv_prandtl);
     }
@@ -530,6 +532,7 @@ bool select_fast(const DevGeom& G, const spd_flux_desc& f, int& keep, int& pd, i
             C.mu_ref_s = sc*t.mu; C.T_ref = t.T_ref; C.power = t.power; C.prandtl = t.prandtl;
             C.inv_T_ref = 1.0/t.T_ref; C.inv_prandtl = 1.0/t.prandtl;
             C.t_ref_one = (t.T_ref == 1.0);
+            C.ps = PowSeries::make(t.power);
             if (t.law == SPD_LAW_POWER && !(t.T_ref > 0.0 && t.prandtl > 0.0)) return false;
         }
         else return false;
