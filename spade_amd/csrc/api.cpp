@@ -111,9 +111,11 @@ spd_array_s* new_array(Grid& g)
     auto* A = new spd_array_s;
     std::unique_ptr<spd_array_s> guard(A);
     A->a.grid = &g;
-    const size_t bytes = sizeof(double)*(size_t)g.geom.sb*g.geom.nblk;
-    SPD_HIP(hipMalloc((void**)&A->a.d, bytes));
-    SPD_HIP(hipMemsetAsync(A->a.d, 0, bytes, g.ctx->stream));
+    const int lead = lead_doubles(g.geom);
+    const size_t bytes = sizeof(double)*((size_t)g.geom.sb*g.geom.nblk + lead);
+    SPD_HIP(hipMalloc((void**)&A->a.base, bytes));
+    A->a.d = A->a.base + lead;
+    SPD_HIP(hipMemsetAsync(A->a.base, 0, bytes, g.ctx->stream));
     return guard.release();
 }
 
@@ -121,7 +123,7 @@ void free_array(spd_array_s* A)
 {
     if (!A) return;
     SPD_HIP(hipStreamSynchronize(A->a.grid->ctx->stream));
-    SPD_HIP(hipFree(A->a.d));
+    SPD_HIP(hipFree(A->a.base));
     delete A;
 }
 
@@ -433,7 +435,7 @@ int spd_integrator_create(spd_array_t q, const spd_flux_desc* flux, const spd_bc
         }
         catch (...)
         {
-            for (auto*& r : it->r) { if (r) { (void)hipFree(r->a.d); delete r; r = nullptr; } }
+            for (auto*& r : it->r) { if (r) { (void)hipFree(r->a.base); delete r; r = nullptr; } }
             throw;
         }
         *out = guard.release();

@@ -28,6 +28,12 @@ namespace {
 
 constexpr int HXB = 18, HPB = 18*18;
 
+// planes of conserved state in flight ahead of the one entering the LDS window
+#ifndef SPD_BATCH_PREFETCH
+#define SPD_BATCH_PREFETCH 1
+#endif
+constexpr int PFB = SPD_BATCH_PREFETCH;
+
 struct CellF { float p, T, u, v, w, rho, e; };   // p, T, rho, e are perturbations
 
 __device__ inline int lposb(int i, int j) { return (j + 1)*HXB + (i + 1); }
@@ -183,12 +189,14 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             return CellF{t[p], t[HPB + p], s0[p], s0[HPB + p], s0[2*HPB + p], t[2*HPB + p], t[3*HPB + p]};
         };
 
-        // prologue: planes -1, 0, 1 resident, plane 2 in flight
+        // prologue: planes -1, 0, 1 resident, planes 2 .. 1 + PFB in flight
         const PrimD qprev = to_prim(P, load(i, j, -1));
         PrimD qcur = to_prim(P, load(i, j, 0));
         PrimD qnxt = to_prim(P, load(i, j, 1));
         CellF prev = to_f32(P, qprev), cur = to_f32(P, qcur), nxt = to_f32(P, qnxt);
-        Cons nraw = load(i, j, 2), hraw{};
+        Cons nraw[PFB], hraw[PFB];
+        #pragma unroll
+        for (int d = 0; d < PFB; ++d) { nraw[d] = load(i, j, 2 + d); hraw[d] = Cons{}; }
         {
             float* s = sU[slotk(-1)];
             s[pown] = prev.u; s[HPB + pown] = prev.v; s[2*HPB + pown] = prev.w;
@@ -202,7 +210,8 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             const PrimD h0 = to_prim(P, load(hi, hj, 0)), h1 = to_prim(P, load(hi, hj, 1));
             put(0, phalo, h0, to_f32(P, h0));
             put(1, phalo, h1, to_f32(P, h1));
-            hraw = load(hi, hj, 2);
+            #pragma unroll
+            for (int d = 0; d < PFB; ++d) hraw[d] = load(hi, hj, 2 + d);
         }
         double ke_acc = 0.0, en_acc = 0.0;
         double wprev_u = qprev.u, wprev_v = qprev.v, wprev_w = qprev.w;   // own column, plane k-1 (f64)
@@ -217,6 +226,15 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
 
         for (int k = 0; k < 16; ++k)
         {
+            // this plane's stage inputs, requested before the face work hides their latency
+            const long long o = (long long)slot*5*4096 + (k*16 + j)*16 + i;
+            double uin[5], u0[5];
+            #pragma unroll
+            for (int n = 0; n < 5; ++n)
+            {
+                uin[n] = Qin[o + n*4096];
+                u0[n] = (alpha == 0.0) ? 0.0 : Q0[o + n*4096];
+            }
             const float* sm = sU[slotk(k - 1)];
             const float* s0 = sU[slotk(k)];
             const float* sp = sU[slotk(k + 1)];
@@ -298,13 +316,11 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             const double d_rho = r[0] + P.rho_ref*DU;
             const double d_E   = r[1] + P.e_ref*d_rho + P.p_ref*DU;
             const double res[5] = {d_rho, d_E, r[2], r[3], r[4]};
-            const long long o = (long long)slot*5*4096 + (k*16 + j)*16 + i;
             #pragma unroll
             for (int n = 0; n < 5; ++n)
             {
-                const double uin = Qin[o + n*4096];
-                const double stage = uin + P.dt*res[n];
-                Qout[o + n*4096] = (alpha == 0.0) ? beta*stage : alpha*Q0[o + n*4096] + beta*stage;
+                const double stage = uin[n] + P.dt*res[n];
+                Qout[o + n*4096] = (alpha == 0.0) ? beta*stage : alpha*u0[n] + beta*stage;
             }
 
             // advance the window
@@ -312,15 +328,19 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             if (k + 1 < 16)
             {
                 qcur = qnxt; cur = nxt;
-                qnxt = to_prim(P, nraw);
+                qnxt = to_prim(P, nraw[0]);
                 nxt = to_f32(P, qnxt);
                 put(k + 2, pown, qnxt, nxt);
-                if (k + 3 <= 16) nraw = load(i, j, k + 3);
+                #pragma unroll
+                for (int d = 0; d + 1 < PFB; ++d) nraw[d] = nraw[d + 1];
+                if (k + 2 + PFB <= 16) nraw[PFB - 1] = load(i, j, k + 2 + PFB);
                 if (has_halo)
                 {
-                    const PrimD hq = to_prim(P, hraw);
+                    const PrimD hq = to_prim(P, hraw[0]);
                     put(k + 2, phalo, hq, to_f32(P, hq));
-                    if (k + 3 <= 16) hraw = load(hi, hj, k + 3);
+                    #pragma unroll
+                    for (int d = 0; d + 1 < PFB; ++d) hraw[d] = hraw[d + 1];
+                    if (k + 2 + PFB <= 16) hraw[PFB - 1] = load(hi, hj, k + 2 + PFB);
                 }
             }
             __syncthreads();

@@ -31,6 +31,13 @@ namespace {
 
 constexpr int HX = 18, HP = 18*18;
 
+// planes of primitives in flight ahead of the one entering the LDS window:
+// enough bytes in flight per CU to cover HBM latency at full bandwidth
+#ifndef SPD_FAST_PREFETCH
+#define SPD_FAST_PREFETCH 2
+#endif
+constexpr int PF = SPD_FAST_PREFETCH;
+
 struct FastCfg
 {
     double R, gm1;                  // gas constant, gamma - 1
@@ -351,12 +358,18 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
         t[p] = c.p; t[HP + p] = c.T; t[2*HP + p] = c.rho; t[3*HP + p] = c.e;
     };
 
-    // prologue: planes -1, 0, 1 resident, plane 2 in flight
+    // prologue: planes -1, 0, 1 resident, planes 2 .. 1 + PF in flight
     const CellV prev = cell(load_raw(i, j, -1));
     CellV cur = cell(load_raw(i, j, 0));
     CellV nxt = cell(load_raw(i, j, 1));
-    Raw nraw{}, hraw{};
-    if (2 <= nz) nraw = load_raw(i, j, 2);
+    Raw nraw[PF], hraw[PF];
+    #pragma unroll
+    for (int d = 0; d < PF; ++d)
+    {
+        nraw[d] = Raw{};
+        hraw[d] = Raw{};
+        if (2 + d <= nz) nraw[d] = load_raw(i, j, 2 + d);
+    }
     {
         double* s = sU[slot(-1)];
         s[pown] = prev.u; s[HP + pown] = prev.v; s[2*HP + pown] = prev.w;
@@ -369,7 +382,9 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
         s[phalo] = hp.u; s[HP + phalo] = hp.v; s[2*HP + phalo] = hp.w;
         put(0, phalo, cell(load_raw(hi, hj, 0)));
         put(1, phalo, cell(load_raw(hi, hj, 1)));
-        if (2 <= nz) hraw = load_raw(hi, hj, 2);
+        #pragma unroll
+        for (int d = 0; d < PF; ++d)
+            if (2 + d <= nz) hraw[d] = load_raw(hi, hj, 2 + d);
     }
     __syncthreads();
 
@@ -455,15 +470,19 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
         if (k + 1 < nz)
         {
             // plane k+2 (<= nz) enters from the prefetched values into the ring
-            // slot of plane k-1 and the parity buffer of plane k; plane k+3 is requested
+            // slot of plane k-1 and the parity buffer of plane k; plane k+2+PF is requested
             cur = nxt;
-            nxt = cell(nraw);
+            nxt = cell(nraw[0]);
             put(k + 2, pown, nxt);
-            if (k + 3 <= nz) nraw = load_raw(i, j, k + 3);
+            #pragma unroll
+            for (int d = 0; d + 1 < PF; ++d) nraw[d] = nraw[d + 1];
+            if (k + 2 + PF <= nz) nraw[PF - 1] = load_raw(i, j, k + 2 + PF);
             if (has_halo)
             {
-                put(k + 2, phalo, cell(hraw));
-                if (k + 3 <= nz) hraw = load_raw(hi, hj, k + 3);
+                put(k + 2, phalo, cell(hraw[0]));
+                #pragma unroll
+                for (int d = 0; d + 1 < PF; ++d) hraw[d] = hraw[d + 1];
+                if (k + 2 + PF <= nz) hraw[PF - 1] = load_raw(hi, hj, k + 2 + PF);
             }
         }
         __syncthreads();

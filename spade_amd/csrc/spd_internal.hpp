@@ -64,8 +64,20 @@ struct Grid
 struct Array
 {
     Grid* grid = nullptr;
-    double* d = nullptr;          // device data, geom.sb * nblk doubles
+    double* base = nullptr;       // allocation
+    double* d = nullptr;          // device data, geom.sb * nblk doubles, at base + lead_doubles(geom)
 };
+
+// Offset of an array's data from its 256-byte aligned allocation, chosen so
+// that the interior part of every row (i = 0 .. nc0-1) starts on a 32-byte
+// boundary: with a row stride N0 that is a multiple of 4 doubles, a lead of
+// -ng mod 4 doubles aligns all of them, and whole-row stores of the interior
+// then cover full 32-byte sectors instead of partial ones at both ends.
+inline int lead_doubles(const DevGeom& g)
+{
+    if (g.N[0] % 4 != 0) return 0;
+    return (4 - g.ng % 4) % 4;
+}
 
 // ---- kernels' host launchers ----
 void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f,
