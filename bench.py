@@ -118,10 +118,30 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--blocks", type=int, default=16, help="blocks per axis (16 -> 256^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-chosen-form", action="store_true",
+                    help="skip the batched / served FP32-flux measurements")
+    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--fp64-sample", type=int, default=12, help="ensemble cases timed on the FP64 path")
+    ap.add_argument("--ensemble-cases", type=int, default=300)
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--serve-requests", type=int, default=60)
+    ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate, requests/s")
     ap.add_argument("--copy-check", type=int, default=0,
                     help="after the timed loop, copy the primitive array N times (a plain device copy of "
                          "known size, used to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="instead of the benchmark: FP32-flux vs FP64 TGV run (errors every 100 steps)")
+    ap.add_argument("--study-cells", type=int, default=256)
+    ap.add_argument("--study-steps", type=int, default=1000)
     args = ap.parse_args()
+    if args.precision_study:
+        import spade_amd as sa
+        from spade_amd import studies
+        sa.load()
+        res = studies.precision(args.study_cells, args.study_steps, 100,
+                                log=lambda r: print(json.dumps(r), file=sys.stderr, flush=True))
+        print(json.dumps(res), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the path shards by blocks "
               "across GPUs but no multi-GPU run is part of this build", file=sys.stderr)
@@ -185,10 +205,49 @@ def main():
         # the device array of this geometry is the host image without padding:
         # 20^3 doubles per variable is a multiple of the 256-byte variable stride
         line["copy_check_bytes"] = 8 * int(np.prod(g.host_shape))
+    del q, rhs
+    if not args.no_chosen_form:
+        line["chosen_form"] = chosen_form(args, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu_baseline:
         cb = cpu_baseline()
         line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
+
+
+def chosen_form(args, log):
+    """Batched and served FP32-flux form against the plain FP64 path (cell-updates/s)."""
+    from spade_amd import studies
+    out = {}
+    # plain FP64 single-case path: 256^3 TGV, ssprk3 steps (3 exchanges, 3 flux_div, 3 stage updates)
+    from spade_amd import CaseSpec
+    big = CaseSpec("tgv", (256,) * 3, mach=0.1, reynolds=1600.0)
+    cu, sec = studies.fp64_rate(big, args.fp64_steps)
+    out["fp64_single_case"] = {"workload": "TGV 256^3, one case, FP64 flux_div + transform-aware SSP-RK3",
+                               "cell_updates_per_s": cu / sec, "ms_per_step": sec / args.fp64_steps * 1e3}
+    log(f"fp64 single case {cu / sec / 1e9:.3f} Gcell-updates/s")
+    # ensemble: 300 cases, all resident, persistent stage kernels + hipGraph + tuned launch shape
+    specs = studies.ensemble_specs(args.ensemble_cases, args.ensemble_steps)
+    ens = studies.run_ensemble(specs, args.ensemble_steps)
+    log(f"ensemble {ens['cell_updates_per_s'] / 1e9:.3f} Gcell-updates/s")
+    # the FP64 path on a sample of the same cases, one at a time
+    tot_cu, tot_s = 0, 0.0
+    for sp in specs[:args.fp64_sample]:
+        cu, sec = studies.fp64_rate(sp, args.fp64_steps)
+        tot_cu += cu
+        tot_s += sec
+    fp64_ens = tot_cu / tot_s
+    ens.update({"fp64_same_cases_cell_updates_per_s": fp64_ens, "fp64_sample_cases": args.fp64_sample,
+                "speedup_vs_fp64_same_cases": ens["cell_updates_per_s"] / fp64_ens,
+                "speedup_vs_fp64_256": ens["cell_updates_per_s"] / out["fp64_single_case"]["cell_updates_per_s"]})
+    out["ensemble"] = ens
+    log(f"fp64 on ensemble cases {fp64_ens / 1e9:.3f} Gcell-updates/s")
+    # serving: Poisson arrivals, continuous batching
+    srv, _ = studies.serving(args.serve_requests, args.serve_rate)
+    srv["speedup_vs_fp64_same_cases"] = srv["cell_updates_per_s"] / fp64_ens
+    out["served"] = srv
+    log(f"served {srv['cell_updates_per_s'] / 1e9:.3f} Gcell-updates/s, p50 {srv['p50_latency_s']:.3f}s "
+        f"p99 {srv['p99_latency_s']:.3f}s")
+    return out
 
 
 if __name__ == "__main__":
