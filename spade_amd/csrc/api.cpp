@@ -271,10 +271,13 @@ int spd_grid_create(spd_ctx_t ctx, const spd_grid_desc* d, spd_grid_t* out)
         SPD_HIP(hipSetDevice(ctx->c.device));
         SPD_HIP(hipMalloc((void**)&g.d_tables, sizeof(double)*9*dg.nblk));
         SPD_HIP(hipMalloc((void**)&g.d_nbr, sizeof(int)*27*dg.nblk));
+        SPD_HIP(hipMalloc((void**)&g.d_start, sizeof(int)*dg.nblk));
         SPD_HIP(hipMemcpy(g.d_tables, g.h_dx.data(), sizeof(double)*3*dg.nblk, hipMemcpyHostToDevice));
         SPD_HIP(hipMemcpy(g.d_tables + 3*dg.nblk, g.h_inv_dx.data(), sizeof(double)*3*dg.nblk, hipMemcpyHostToDevice));
         SPD_HIP(hipMemcpy(g.d_tables + 6*dg.nblk, g.h_bmin.data(), sizeof(double)*3*dg.nblk, hipMemcpyHostToDevice));
         SPD_HIP(hipMemcpy(g.d_nbr, nbr.data(), sizeof(int)*27*dg.nblk, hipMemcpyHostToDevice));
+        const std::vector<int> complete(dg.nblk, dg.nc[2]);
+        SPD_HIP(hipMemcpy(g.d_start, complete.data(), sizeof(int)*dg.nblk, hipMemcpyHostToDevice));
         dg.dx = g.d_tables;
         dg.inv_dx = g.d_tables + 3*dg.nblk;
         dg.bmin = g.d_tables + 6*dg.nblk;
@@ -290,6 +293,7 @@ int spd_grid_destroy(spd_grid_t grid)
         SPD_HIP(hipStreamSynchronize(grid->g.ctx->stream));
         SPD_HIP(hipFree(grid->g.d_tables));
         SPD_HIP(hipFree(grid->g.d_nbr));
+        SPD_HIP(hipFree(grid->g.d_start));
         delete grid;
     });
 }

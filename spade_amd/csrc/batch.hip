@@ -136,7 +136,7 @@ __device__ inline CellF to_f32(const CaseParams& P, const PrimD& q)
                  (float)(q.rho - P.rho_ref), (float)(P.cv*Tp)};
 }
 
-template <int LAW>
+template <int LAW, bool HAS_U0>
 __global__ void __launch_bounds__(256, 2)
 k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q0, double* __restrict__ Qout,
         double alpha, double beta, int want_diag)
@@ -189,6 +189,21 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             return CellF{t[p], t[HPB + p], s0[p], s0[HPB + p], s0[2*HPB + p], t[2*HPB + p], t[3*HPB + p]};
         };
 
+        // stage inputs of the plane to be written next, requested before the
+        // primitive prefetch so that waiting for them never waits for it
+        double uin[5], u0[5];
+        auto load_stage_inputs = [&](int k)
+        {
+            const long long o = (long long)slot*5*4096 + (k*16 + j)*16 + i;
+            #pragma unroll
+            for (int n = 0; n < 5; ++n)
+            {
+                uin[n] = Qin[o + n*4096];
+                u0[n] = HAS_U0 ? Q0[o + n*4096] : 0.0;
+            }
+        };
+        load_stage_inputs(0);
+
         // prologue: planes -1, 0, 1 resident, planes 2 .. 1 + PFB in flight
         const PrimD qprev = to_prim(P, load(i, j, -1));
         PrimD qcur = to_prim(P, load(i, j, 0));
@@ -215,7 +230,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
         }
         double ke_acc = 0.0, en_acc = 0.0;
         double wprev_u = qprev.u, wprev_v = qprev.v, wprev_w = qprev.w;   // own column, plane k-1 (f64)
-        __syncthreads();
+        lds_barrier();
 
         float zlo[5];
         {
@@ -226,15 +241,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
 
         for (int k = 0; k < 16; ++k)
         {
-            // this plane's stage inputs, requested before the face work hides their latency
             const long long o = (long long)slot*5*4096 + (k*16 + j)*16 + i;
-            double uin[5], u0[5];
-            #pragma unroll
-            for (int n = 0; n < 5; ++n)
-            {
-                uin[n] = Qin[o + n*4096];
-                u0[n] = (alpha == 0.0) ? 0.0 : Q0[o + n*4096];
-            }
             const float* sm = sU[slotk(k - 1)];
             const float* s0 = sU[slotk(k)];
             const float* sp = sU[slotk(k + 1)];
@@ -300,7 +307,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 ke_acc += 0.5*qcur.rho*(qcur.u*qcur.u + qcur.v*qcur.v + qcur.w*qcur.w);
                 en_acc += 0.5*(ox*ox + oy*oy + oz*oz);
             }
-            __syncthreads();
+            lds_barrier();
 
             // FP64 residual sum and stage combination for cell (i,j,k)
             const double fi[3] = {P.ix, P.iy, P.iz};
@@ -320,13 +327,14 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             for (int n = 0; n < 5; ++n)
             {
                 const double stage = uin[n] + P.dt*res[n];
-                Qout[o + n*4096] = (alpha == 0.0) ? beta*stage : alpha*u0[n] + beta*stage;
+                Qout[o + n*4096] = HAS_U0 ? alpha*u0[n] + beta*stage : beta*stage;
             }
 
             // advance the window
             wprev_u = qcur.u; wprev_v = qcur.v; wprev_w = qcur.w;
             if (k + 1 < 16)
             {
+                load_stage_inputs(k + 1);
                 qcur = qnxt; cur = nxt;
                 qnxt = to_prim(P, nraw[0]);
                 nxt = to_f32(P, qnxt);
@@ -343,16 +351,16 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                     if (k + 2 + PFB <= 16) hraw[PFB - 1] = load(hi, hj, k + 2 + PFB);
                 }
             }
-            __syncthreads();
+            lds_barrier();
         }
         if (want_diag)
         {
             sRed[0][tid] = ke_acc; sRed[1][tid] = en_acc;
-            __syncthreads();
+            lds_barrier();
             for (int s = 128; s > 0; s >>= 1)
             {
                 if (tid < s) { sRed[0][tid] += sRed[0][tid + s]; sRed[1][tid] += sRed[1][tid + s]; }
-                __syncthreads();
+                lds_barrier();
             }
             if (tid == 0)
             {
@@ -360,7 +368,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 B.diag[((long long)h*B.max_slots + slot)*2 + 0] = sRed[0][0];
                 B.diag[((long long)h*B.max_slots + slot)*2 + 1] = sRed[1][0];
             }
-            __syncthreads();
+            lds_barrier();
         }
     }
 }
@@ -418,10 +426,18 @@ k_init_case(BatchDev B, const int* __restrict__ slots, int nslots, CaseParams P,
 void launch_batch_stage(const BatchDev& B, int law, const double* Qin, const double* Q0, double* Qout,
                         double alpha, double beta, int want_diag, int grid, hipStream_t s)
 {
+    // stage 1 of the Shu-Osher form (alpha = 0) reads no u0
+    const bool u0 = alpha != 0.0;
     if (law == SPD_LAW_POWER)
-        hipLaunchKernelGGL(k_stage<SPD_LAW_POWER>, dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag);
+    {
+        if (u0) hipLaunchKernelGGL((k_stage<SPD_LAW_POWER, true>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag);
+        else    hipLaunchKernelGGL((k_stage<SPD_LAW_POWER, false>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag);
+    }
     else
-        hipLaunchKernelGGL(k_stage<SPD_LAW_CONST>, dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag);
+    {
+        if (u0) hipLaunchKernelGGL((k_stage<SPD_LAW_CONST, true>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag);
+        else    hipLaunchKernelGGL((k_stage<SPD_LAW_CONST, false>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag);
+    }
     SPD_LAUNCH_CHECK();
 }
 

@@ -31,12 +31,6 @@ namespace {
 
 constexpr int HX = 18, HP = 18*18;
 
-// planes of primitives in flight ahead of the one entering the LDS window:
-// enough bytes in flight per CU to cover HBM latency at full bandwidth
-#ifndef SPD_FAST_PREFETCH
-#define SPD_FAST_PREFETCH 2
-#endif
-constexpr int PF = SPD_FAST_PREFETCH;
 
 struct FastCfg
 {
@@ -73,14 +67,13 @@ __device__ inline double div_const(double a, double b, double y)
     return __builtin_fma(r, y, q0);
 }
 
-// library pow, kept out of line: it only runs for T/T_ref far from 1
-__device__ __attribute__((noinline)) double pow_far(double x, double n) { return pow(x, n); }
-
-// x^n of the power law: binomial series near 1, library pow otherwise
-__device__ inline double pow_law(const FastCfg& C, double x)
+// x^n of the power law by the branch-free binomial series; `bad` records a
+// lane whose x is outside the series range (its block is then completed by
+// the fix-up kernel with the library pow, from that plane on)
+__device__ inline double pow_law(const FastCfg& C, double x, bool& bad)
 {
-    if (C.ps.in_range(x)) return C.ps.eval(x);
-    return pow_far(x, C.power);
+    bad |= !C.ps.in_range(x);
+    return C.ps.eval(x);
 }
 
 // ducros_t::get_sensor (state_sensor.h:32-42); g[c][q], q = u, v, w
@@ -135,14 +128,15 @@ __device__ inline void pdiss_s(const FastCfg& C, const CellV& L, const CellV& R,
 // visc_lr row D for the unit normal of identity coordinates (viscous.h:39-112),
 // times the composition scale s (carried by mu, beta and the conductivity).
 template <int D, int LAW>
-__device__ inline void visc_s(const FastCfg& C, const FaceSums& f, const double g[3][3], double gT, double F[5])
+__device__ inline void visc_s(const FastCfg& C, const FaceSums& f, const double g[3][3], double gT, double F[5],
+                              bool& bad)
 {
     double mu, visc2, cond;
     if (LAW == SPD_LAW_POWER)
     {
         const double Tf = f.ST*0.5;
         const double x = C.t_ref_one ? Tf : div_const(Tf, C.T_ref, C.inv_T_ref);
-        mu    = C.mu_ref_s*pow_law(C, x);
+        mu    = C.mu_ref_s*pow_law(C, x, bad);
         visc2 = -0.66666666667*mu;
         cond  = C.spec_heat*div_const(mu, C.prandtl, C.inv_prandtl);
     }
@@ -191,11 +185,11 @@ template <int KEEP, int PDISS, int VISC> struct Spec
 // into 1/dx.
 template <int D, class S>
 __device__ inline void face_product(const FastCfg& C, const CellV& L, const CellV& R, const double g[3][3], double gT,
-                                    double idx, double P[5])
+                                    double idx, double P[5], bool& bad)
 {
     const FaceSums f = face_sums(L, R);
     double s8[5];
-    if (S::vs) visc_s<D, S::law>(C, f, g, gT, s8);
+    if (S::vs) visc_s<D, S::law>(C, f, g, gT, s8, bad);
     if (S::pd)
     {
         double z[5];
@@ -307,11 +301,13 @@ __device__ inline int lpos(int i, int j) { return (j + 1)*HX + (i + 1); }
 
 struct Raw { double p, T, u, v, w; };
 
-template <int KEEP, int PDISS, int VISC>
+template <int KEEP, int PDISS, int VISC, bool INC>
 __global__ void __launch_bounds__(256, 2)
-k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, FastCfg C, int increment)
+k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, FastCfg C,
+            int* __restrict__ start)
 {
     using S = Spec<KEEP, PDISS, VISC>;
+    __shared__ int sBad[2];               // a face of plane k is outside the pow series range (by parity)
     __shared__ double sU[3][3*HP];        // u, v, w of planes k-1, k, k+1 (ring)
     __shared__ double sS[2][4*HP];        // p, T, rho, e of planes k, k+1 (by parity)
     __shared__ double sFx[5][16*17];      // x-face products, faces i = 0..16 of row j
@@ -320,12 +316,11 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     const int lb = blockIdx.x;
     const int tid = threadIdx.x;
     const int i = tid & 15, j = tid >> 4;
-    const double* __restrict__ B = prims + (long long)lb*G.sb;
-    double* __restrict__ Rb = rhs + (long long)lb*G.sb;
     const double ix = G.inv_dx[3*lb + 0], iy = G.inv_dx[3*lb + 1], iz = G.inv_dx[3*lb + 2];
     const int nz = G.nc[2];
     const long long sv = G.sv;
     const int N0 = G.N[0], N1 = G.N[1], ng = G.ng;
+    const long long plane = (long long)N0*N1;
 
     // halo cell of this thread: waves 0 and 1 own the 64 edge cells and the
     // 4 corners of the 18x18 plane (waves 2 and 3 evaluate the block-boundary faces)
@@ -338,162 +333,201 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     else if (tid < 68) { const int c = tid - 64; hi = (c & 1) ? 16 : -1; hj = (c & 2) ? 16 : -1; }
     const int pown = lpos(i, j), phalo = lpos(hi, hj);
 
-    auto gidx = [&](int ci, int cj, int ck) -> long long
+    // column bases at plane k = 0 (stored plane ng); plane k is base + k*plane
+    const long long blk = (long long)lb*G.sb;
+    const double* __restrict__ Bown = prims + blk + ((long long)ng*N1 + (j + ng))*N0 + (i + ng);
+    const double* __restrict__ Bhalo = prims + blk + ((long long)ng*N1 + (hj + ng))*N0 + (hi + ng);
+    double* __restrict__ Rown = rhs + blk + ((long long)ng*N1 + (j + ng))*N0 + (i + ng);
+
+    auto load_at = [&](const double* __restrict__ col, int ck) -> Raw
     {
-        return ((long long)(ck + ng)*N1 + (cj + ng))*N0 + (ci + ng);
-    };
-    auto load_raw = [&](int ci, int cj, int ck) -> Raw
-    {
-        const long long o = gidx(ci, cj, ck);
-        return Raw{B[o], B[sv + o], B[2*sv + o], B[3*sv + o], B[4*sv + o]};
+        const double* e = col + ck*plane;
+        return Raw{e[0], e[sv], e[2*sv], e[3*sv], e[4*sv]};
     };
     auto cell = [&](const Raw& r) -> CellV { return make_cell(C, r.p, r.T, r.u, r.v, r.w); };
-    auto slot = [](int k) { return (k + 3) % 3; };
-    // a plane entering the window: velocities to the ring, scalars to its parity buffer
-    auto put = [&](int k, int p, const CellV& c)
+    // LDS window: ring slots of velocities (rotated by pointer), scalar buffers by parity
+    double* ring[3] = {sU[0], sU[1], sU[2]};      // planes k-1, k, k+1
+    auto put_vel = [&](double* s, int p, const CellV& c) { s[p] = c.u; s[HP + p] = c.v; s[2*HP + p] = c.w; };
+    auto put_sca = [&](double* t, int p, const CellV& c)
     {
-        double* s = sU[slot(k)];
-        s[p] = c.u; s[HP + p] = c.v; s[2*HP + p] = c.w;
-        double* t = sS[k & 1];
         t[p] = c.p; t[HP + p] = c.T; t[2*HP + p] = c.rho; t[3*HP + p] = c.e;
     };
 
-    // prologue: planes -1, 0, 1 resident, planes 2 .. 1 + PF in flight
-    const CellV prev = cell(load_raw(i, j, -1));
-    CellV cur = cell(load_raw(i, j, 0));
-    CellV nxt = cell(load_raw(i, j, 1));
-    Raw nraw[PF], hraw[PF];
+    // increment mode: the residual of the next plane to be written is requested
+    // before that iteration's primitive prefetch, so waiting for it never waits
+    // for the prefetch
+    double rin[5];
     #pragma unroll
-    for (int d = 0; d < PF; ++d)
-    {
-        nraw[d] = Raw{};
-        hraw[d] = Raw{};
-        if (2 + d <= nz) nraw[d] = load_raw(i, j, 2 + d);
-    }
-    {
-        double* s = sU[slot(-1)];
-        s[pown] = prev.u; s[HP + pown] = prev.v; s[2*HP + pown] = prev.w;
-    }
-    put(0, pown, cur); put(1, pown, nxt);
+    for (int n = 0; n < 5; ++n) rin[n] = INC ? Rown[n*sv] : 0.0;
+
+    // prologue: planes -1, 0, 1 resident; planes 2 and 3 in flight (two raw
+    // register sets used alternately, so a plane's load is never copied and
+    // waiting for it never waits for the later one)
+    const CellV prev = cell(load_at(Bown, -1));
+    const CellV cur0 = cell(load_at(Bown, 0));
+    const CellV nxt0 = cell(load_at(Bown, 1));
+    Raw nA{}, nB{}, hA{}, hB{};
+    if (2 <= nz) nA = load_at(Bown, 2);
+    if (3 <= nz) nB = load_at(Bown, 3);
+    put_vel(ring[0], pown, prev);
+    put_vel(ring[1], pown, cur0); put_sca(sS[0], pown, cur0);
+    put_vel(ring[2], pown, nxt0); put_sca(sS[1], pown, nxt0);
+    if (tid == 0) { sBad[0] = 0; sBad[1] = 0; }
     if (has_halo)
     {
-        const CellV hp = cell(load_raw(hi, hj, -1));
-        double* s = sU[slot(-1)];
-        s[phalo] = hp.u; s[HP + phalo] = hp.v; s[2*HP + phalo] = hp.w;
-        put(0, phalo, cell(load_raw(hi, hj, 0)));
-        put(1, phalo, cell(load_raw(hi, hj, 1)));
-        #pragma unroll
-        for (int d = 0; d < PF; ++d)
-            if (2 + d <= nz) hraw[d] = load_raw(hi, hj, 2 + d);
+        put_vel(ring[0], phalo, cell(load_at(Bhalo, -1)));
+        const CellV h0 = cell(load_at(Bhalo, 0)), h1 = cell(load_at(Bhalo, 1));
+        put_vel(ring[1], phalo, h0); put_sca(sS[0], phalo, h0);
+        put_vel(ring[2], phalo, h1); put_sca(sS[1], phalo, h1);
+        if (2 <= nz) hA = load_at(Bhalo, 2);
+        if (3 <= nz) hB = load_at(Bhalo, 3);
     }
-    __syncthreads();
+    lds_barrier();
 
     double zlo[5];
+    bool bad = false;                     // carried into plane 0's check
     {
         double g[3][3], gT;
-        grads_z<S>(sU[slot(-1)], sU[slot(0)], pown, prev, cur, ix, iy, iz, g, gT);
-        face_product<2, S>(C, prev, cur, g, gT, iz, zlo);
+        grads_z<S>(ring[0], ring[1], pown, prev, cur0, ix, iy, iz, g, gT);
+        face_product<2, S>(C, prev, cur0, g, gT, iz, zlo, bad);
     }
 
-    for (int k = 0; k < nz; ++k)
+    // One plane k: every face, the residual of its cells, and the entry of
+    // plane k+2 (from nr / hr, which are then refilled with plane k+4).
+    // Returns false when the block must be left to the fix-up kernel.
+    auto plane_step = [&](int k, Raw& nr, Raw& hr) -> bool
     {
-        const double* sm = sU[slot(k - 1)];
-        const double* s0 = sU[slot(k)];
-        const double* sp = sU[slot(k + 1)];
+        const double* sm = ring[0];
+        const double* s0 = ring[1];
+        const double* sp = ring[2];
         const double* sk = sS[k & 1];
-        // low x-face of (i,j): L = (i-1, j)
+        // the cells of this column in planes k and k+1, from the window
+        const CellV cur = cell_from_lds(sk, s0, pown);
+        const CellV nxt = cell_from_lds(sS[(k + 1) & 1], sp, pown);
+        double zhi[5];
+        // the three faces of all lanes, then the block-boundary x-faces
+        // (wave 2) and y-faces (wave 3); the series pow has no branch, so the
+        // three faces form one straight-line block
         {
+            // low x-face of (i,j): L = (i-1, j)
             const int pL = pown - 1;
             const CellV L = cell_from_lds(sk, s0, pL);
             double g[3][3], gT, P[5];
             grads_x<S>(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
-            face_product<0, S>(C, L, cur, g, gT, ix, P);
+            face_product<0, S>(C, L, cur, g, gT, ix, P, bad);
             for (int n = 0; n < 5; ++n) sFx[n][j*17 + i] = P[n];
         }
-        // low y-face of (i,j): L = (i, j-1)
         {
+            // low y-face of (i,j): L = (i, j-1)
             const int pL = pown - HX;
             const CellV L = cell_from_lds(sk, s0, pL);
             double g[3][3], gT, P[5];
             grads_y<S>(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
-            face_product<1, S>(C, L, cur, g, gT, iy, P);
+            face_product<1, S>(C, L, cur, g, gT, iy, P, bad);
             for (int n = 0; n < 5; ++n) sFy[n][j*16 + i] = P[n];
         }
-        // block-boundary x-faces (16 - 1/2, jj) on wave 2
+        {
+            // high z-face k+1/2: L = (i,j,k), R = (i,j,k+1)
+            double g[3][3], gT;
+            grads_z<S>(s0, sp, pown, cur, nxt, ix, iy, iz, g, gT);
+            face_product<2, S>(C, cur, nxt, g, gT, iz, zhi, bad);
+        }
         if (tid >= 128 && tid < 144)
         {
+            // block-boundary x-faces (16 - 1/2, jj)
             const int jj = tid - 128;
             const int pL = lpos(15, jj), pR = lpos(16, jj);
             const CellV L = cell_from_lds(sk, s0, pL), Rr = cell_from_lds(sk, s0, pR);
             double g[3][3], gT, P[5];
             grads_x<S>(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
-            face_product<0, S>(C, L, Rr, g, gT, ix, P);
+            face_product<0, S>(C, L, Rr, g, gT, ix, P, bad);
             for (int n = 0; n < 5; ++n) sFx[n][jj*17 + 16] = P[n];
         }
-        // block-boundary y-faces (ii, 16 - 1/2) on wave 3
         if (tid >= 192 && tid < 208)
         {
+            // block-boundary y-faces (ii, 16 - 1/2)
             const int ii = tid - 192;
             const int pL = lpos(ii, 15), pR = lpos(ii, 16);
             const CellV L = cell_from_lds(sk, s0, pL), Rr = cell_from_lds(sk, s0, pR);
             double g[3][3], gT, P[5];
             grads_y<S>(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
-            face_product<1, S>(C, L, Rr, g, gT, iy, P);
+            face_product<1, S>(C, L, Rr, g, gT, iy, P, bad);
             for (int n = 0; n < 5; ++n) sFy[n][16*16 + ii] = P[n];
         }
-        // high z-face k+1/2: L = (i,j,k), R = (i,j,k+1)
-        double zhi[5];
+        #pragma unroll
+        for (int n = 0; n < 5; ++n) keep_here(zhi[n]);
+        if (bad) sBad[k & 1] = 1;
+        lds_barrier();
+        if (sBad[k & 1])
         {
-            double g[3][3], gT;
-            grads_z<S>(s0, sp, pown, cur, nxt, ix, iy, iz, g, gT);
-            face_product<2, S>(C, cur, nxt, g, gT, iz, zhi);
+            // some face of plane k needs the library pow: planes k .. nz-1 are
+            // left to the fix-up kernel (nothing of them was written)
+            if (tid == 0) start[lb] = k;
+            return false;
         }
-        __syncthreads();
+        if (tid == 0) sBad[(k + 1) & 1] = 0;
+        bad = false;
 
         // rhs of cell (i,j,k): faces in the order x-, x+, y-, y+, z-, z+
         // (flux_div_basic.h:54-72; the high faces enter as -P)
-        const long long o = gidx(i, j, k);
+        double* __restrict__ ro = Rown + k*plane;
         #pragma unroll
         for (int n = 0; n < 5; ++n)
         {
-            double r = increment ? Rb[n*sv + o] : 0.0;
+            double r = rin[n];
             r += sFx[n][j*17 + i];
             r -= sFx[n][j*17 + i + 1];
             r += sFy[n][j*16 + i];
             r -= sFy[n][(j + 1)*16 + i];
             r += zlo[n];
             r -= zhi[n];
-            Rb[n*sv + o] = r;
+            ro[n*sv] = r;
             zlo[n] = zhi[n];
         }
 
         if (k + 1 < nz)
         {
-            // plane k+2 (<= nz) enters from the prefetched values into the ring
-            // slot of plane k-1 and the parity buffer of plane k; plane k+2+PF is requested
-            cur = nxt;
-            nxt = cell(nraw[0]);
-            put(k + 2, pown, nxt);
-            #pragma unroll
-            for (int d = 0; d + 1 < PF; ++d) nraw[d] = nraw[d + 1];
-            if (k + 2 + PF <= nz) nraw[PF - 1] = load_raw(i, j, k + 2 + PF);
+            // plane k+2 (<= nz) enters into the ring slot of plane k-1 and the
+            // parity buffer of plane k; its raw registers take plane k+4
+            double* s_new = ring[0];
+            ring[0] = ring[1]; ring[1] = ring[2]; ring[2] = s_new;
+            double* t_new = sS[k & 1];
+            if (INC)
+            {
+                #pragma unroll
+                for (int n = 0; n < 5; ++n) rin[n] = ro[plane + n*sv];
+            }
+            const CellV c2 = cell(nr);
+            put_vel(s_new, pown, c2); put_sca(t_new, pown, c2);
+            if (k + 4 <= nz) nr = load_at(Bown, k + 4);
             if (has_halo)
             {
-                put(k + 2, phalo, cell(hraw[0]));
-                #pragma unroll
-                for (int d = 0; d + 1 < PF; ++d) hraw[d] = hraw[d + 1];
-                if (k + 2 + PF <= nz) hraw[PF - 1] = load_raw(hi, hj, k + 2 + PF);
+                const CellV h = cell(hr);
+                put_vel(s_new, phalo, h); put_sca(t_new, phalo, h);
+                if (k + 4 <= nz) hr = load_at(Bhalo, k + 4);
             }
         }
-        __syncthreads();
+        lds_barrier();
+        return true;
+    };
+
+    for (int k = 0; k < nz; k += 2)
+    {
+        if (!plane_step(k, nA, hA)) return;
+        if (k + 1 < nz && !plane_step(k + 1, nB, hB)) return;
     }
+    if (tid == 0) start[lb] = nz;
 }
 
 template <int KEEP, int PDISS, int VISC>
 void launch_fast(const Grid& g, const double* prims, double* rhs, const FastCfg& C, bool increment, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC>), dim3(g.geom.nblk), dim3(256), 0, s,
-                       g.geom, prims, rhs, C, increment ? 1 : 0);
+    if (increment)
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, true>), dim3(g.geom.nblk), dim3(256), 0, s,
+                           g.geom, prims, rhs, C, g.d_start);
+    else
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false>), dim3(g.geom.nblk), dim3(256), 0, s,
+                           g.geom, prims, rhs, C, g.d_start);
     SPD_LAUNCH_CHECK();
 }
 
@@ -574,16 +608,20 @@ bool launch_flux_div_fast(const Grid& g, const double* prims, double* rhs, const
     int keep, pd, vs;
     FastCfg C;
     if (!select_fast(g.geom, f, keep, pd, vs, C)) return false;
+    bool ok = false;
     switch (keep*3 + pd)
     {
-        case 0: return dispatch_visc<0, 0>(vs, g, prims, rhs, C, increment, s);
-        case 1: return dispatch_visc<0, 1>(vs, g, prims, rhs, C, increment, s);
-        case 2: return dispatch_visc<0, 2>(vs, g, prims, rhs, C, increment, s);
-        case 3: return dispatch_visc<1, 0>(vs, g, prims, rhs, C, increment, s);
-        case 4: return dispatch_visc<1, 1>(vs, g, prims, rhs, C, increment, s);
-        case 5: return dispatch_visc<1, 2>(vs, g, prims, rhs, C, increment, s);
+        case 0: ok = dispatch_visc<0, 0>(vs, g, prims, rhs, C, increment, s); break;
+        case 1: ok = dispatch_visc<0, 1>(vs, g, prims, rhs, C, increment, s); break;
+        case 2: ok = dispatch_visc<0, 2>(vs, g, prims, rhs, C, increment, s); break;
+        case 3: ok = dispatch_visc<1, 0>(vs, g, prims, rhs, C, increment, s); break;
+        case 4: ok = dispatch_visc<1, 1>(vs, g, prims, rhs, C, increment, s); break;
+        case 5: ok = dispatch_visc<1, 2>(vs, g, prims, rhs, C, increment, s); break;
     }
-    return false;
+    // power law: blocks with a face temperature outside the pow series range
+    // are completed by the per-cell kernel (an early exit per block when none)
+    if (ok && vs == 1 + SPD_LAW_POWER) launch_flux_div_fixup(g, prims, rhs, f, increment, s);
+    return ok;
 }
 
 } // namespace spd

@@ -497,19 +497,11 @@ __device__ void eval_term(const Cell& C, const FaceIdx& f, const spd_flux_term& 
     }
 }
 
-__global__ void __launch_bounds__(256)
-k_flux_div_generic(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, spd_flux_desc F, int increment)
+// rhs of cell (i, j, k) of block lb in the reference's per-cell order
+// (flux_div_basic.h:47-72)
+__device__ inline void generic_cell(const DevGeom& G, const double* __restrict__ prims, double* __restrict__ rhs,
+                                    const spd_flux_desc& F, int increment, int i, int j, int k, int lb)
 {
-    const int tiles_x = (G.nc[0] + 15)/16, tiles_y = (G.nc[1] + 15)/16;
-    long long w = blockIdx.x;
-    const int tx = (int)(w % tiles_x); w /= tiles_x;
-    const int ty = (int)(w % tiles_y); w /= tiles_y;
-    const int k  = (int)(w % G.nc[2]); w /= G.nc[2];
-    const int lb = (int)w;
-    const int i = tx*16 + (threadIdx.x & 15);
-    const int j = ty*16 + (threadIdx.x >> 4);
-    if (i >= G.nc[0] || j >= G.nc[1]) return;
-
     Cell C{&G, prims + (long long)lb*G.sb, lb};
     double jac = 1.0;
     if (!G.identity)
@@ -542,6 +534,43 @@ k_flux_div_generic(DevGeom G, const double* __restrict__ prims, double* __restri
     for (int v = 0; v < 5; ++v) rhs[cell_index(G, lb, v, i, j, k)] = relem[v];
 }
 
+__global__ void __launch_bounds__(256)
+k_flux_div_generic(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, spd_flux_desc F, int increment)
+{
+    const int tiles_x = (G.nc[0] + 15)/16, tiles_y = (G.nc[1] + 15)/16;
+    long long w = blockIdx.x;
+    const int tx = (int)(w % tiles_x); w /= tiles_x;
+    const int ty = (int)(w % tiles_y); w /= tiles_y;
+    const int k  = (int)(w % G.nc[2]); w /= G.nc[2];
+    const int lb = (int)w;
+    const int i = tx*16 + (threadIdx.x & 15);
+    const int j = ty*16 + (threadIdx.x >> 4);
+    if (i >= G.nc[0] || j >= G.nc[1]) return;
+    generic_cell(G, prims, rhs, F, increment, i, j, k, lb);
+}
+
+// Completes blocks the LDS-staged kernel left at plane start[lb] < nz (a face
+// temperature outside its pow series range): planes start[lb] .. nz-1 of the
+// block, one workgroup per block.  Planes below start[lb] are complete and
+// planes from start[lb] on were not written, so increment mode stays exact.
+__global__ void __launch_bounds__(256)
+k_flux_fixup(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, spd_flux_desc F, int increment,
+             const int* __restrict__ start)
+{
+    const int lb = blockIdx.x;
+    const int nz = G.nc[2];
+    int s = start[lb];
+    if (s >= nz) return;
+    if (s < 0) s = 0;
+    const int per_plane = G.nc[0]*G.nc[1];
+    const int n = per_plane*(nz - s);
+    for (int c = threadIdx.x; c < n; c += blockDim.x)
+    {
+        const int i = c % G.nc[0], j = (c / G.nc[0]) % G.nc[1], k = s + c/per_plane;
+        generic_cell(G, prims, rhs, F, increment, i, j, k, lb);
+    }
+}
+
 } // namespace
 
 void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f,
@@ -550,6 +579,14 @@ void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, co
     const DevGeom& G = g.geom;
     const long long nwg = (long long)((G.nc[0] + 15)/16)*((G.nc[1] + 15)/16)*G.nc[2]*G.nblk;
     hipLaunchKernelGGL(k_flux_div_generic, dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, increment ? 1 : 0);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_flux_div_fixup(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f, bool increment,
+                           hipStream_t s)
+{
+    hipLaunchKernelGGL(k_flux_fixup, dim3(g.geom.nblk), dim3(256), 0, s, g.geom, prims, rhs, f, increment ? 1 : 0,
+                       g.d_start);
     SPD_LAUNCH_CHECK();
 }
 

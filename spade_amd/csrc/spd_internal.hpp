@@ -58,6 +58,7 @@ struct Grid
     DevGeom geom{};
     double* d_tables = nullptr;   // dx, inv_dx, bmin
     int* d_nbr = nullptr;         // [26*lb + n] source block for ghost region n, -1 = not exchanged
+    int* d_start = nullptr;       // [lb] first plane the LDS-staged flux kernel left to the fix-up kernel
     std::vector<double> h_dx, h_inv_dx, h_bmin;
 };
 
@@ -79,9 +80,30 @@ inline int lead_doubles(const DevGeom& g)
     return (4 - g.ng % 4) % 4;
 }
 
+// ---- device helpers shared by the kernels ----
+
+// pins x in a register at this point of the program: keeps the compiler from
+// sinking its computation past a following barrier
+__device__ inline void keep_here(double x) { asm volatile("" : : "v"(x)); }
+__device__ inline void keep_here(float x) { asm volatile("" : : "v"(x)); }
+
+// Workgroup barrier that orders LDS only.  For kernels whose threads exchange
+// data through LDS alone (global memory read-only, or written only at cells
+// no other thread of the launch reads): the barrier does not wait for
+// outstanding global loads, so a prefetch stays in flight across it, which
+// __syncthreads (a fence over all address spaces) would drain.
+__device__ inline void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // ---- kernels' host launchers ----
 void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f,
                              bool increment, hipStream_t s);
+void launch_flux_div_fixup(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f, bool increment,
+                           hipStream_t s);
 bool launch_flux_div_fast(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f,
                           bool increment, hipStream_t s);
 bool flux_div_fast_eligible(const Grid& g, const spd_flux_desc& f);

@@ -101,3 +101,39 @@ def test_fast_walls():
     got, ref = _run((2, 2, 1), (16, 16, 4), COMBOS["keep2_pdiss_const_visc_const"], seed=3,
                     periodic=(True, False, True), walls=(2, 3))
     assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("increment", [False, True], ids=["overwrite", "increment"])
+def test_fast_pow_outside_series_range(increment):
+    """Hot and cold cells (T/T_ref far from 1) force the blocks that see them to
+    leave the planes from the first affected one to the fix-up kernel; every
+    block, and the increment mode, must still match the oracle."""
+    import spade_amd as sa
+    sa.load()
+    nb, nc, ng = (2, 2, 2), (16, 16, 16), 2
+    host, L = _state(nb, nc, ng, 11)
+    inner = host[:, ng:-ng, ng:-ng, ng:-ng]
+    inner[..., 1] = 1.0 + 0.005 * np.random.default_rng(3).standard_normal(inner.shape[:-1])
+    inner[1, 5, 3, 4, 1] = 1.4       # block 1, plane 5
+    inner[2, 0, 0, 0, 1] = 0.6       # block 2, plane 0 (and the ghost plane of its z-neighbour)
+    inner[3, 15, 7, 15, 1] = 1.3     # block 3, last plane, x-boundary column
+    inner[5, :, :, :, 1] = 1.25      # block 5 entirely out of range
+    g = sa.CartesianGrid(nc, nb, ((0.0, L),) * 3, num_exch=ng)
+    q = sa.GridArray(g).upload(host)
+    sa.create_exchange(q).exchange(q)
+    terms = [K2, VP]
+    flux = RawFlux(terms, GAMMA, R)
+    assert sa.flux_div_kernel(q, flux) == "fast"
+    rhs = sa.GridArray(g, fill=0.0)
+    init = None
+    if increment:
+        init = np.random.default_rng(5).standard_normal(g.host_shape)
+        rhs.upload(init)
+    sa.flux_div(q, rhs, flux, increment=increment)
+    got = gi.interior(rhs.download(), ng)
+    og = po.make_grid(nb, nc, ng, (0, 0, 0), (L, L, L))
+    ref = po.flux_div(og, po.Gas(GAMMA, R), po.make_flux(*[po.make_term(**t) for t in terms]), q.download(),
+                      init.copy() if increment else None, increment=increment)
+    err = gi.rel_maxnorm(got, gi.interior(ref, ng))
+    print("pow outside series range", err, flush=True)
+    assert np.all(err <= 1e-12), err
