@@ -545,8 +545,15 @@ k_flux_div_generic(DevGeom G, const double* __restrict__ prims, double* __restri
     const int lb = (int)w;
     const int i = tx*16 + (threadIdx.x & 15);
     const int j = ty*16 + (threadIdx.x >> 4);
+    // geometry and flux descriptor in LDS: the per-cell code indexes them
+    // with run-time values, which on kernel arguments would make every lane
+    // copy them to private (scratch) memory first
+    __shared__ DevGeom sG;
+    __shared__ spd_flux_desc sF;
+    if (threadIdx.x == 0) { sG = G; sF = F; }
+    __syncthreads();
     if (i >= G.nc[0] || j >= G.nc[1]) return;
-    generic_cell(G, prims, rhs, F, increment, i, j, k, lb);
+    generic_cell(sG, prims, rhs, sF, increment, i, j, k, lb);
 }
 
 // Completes blocks the LDS-staged kernel left at plane start[lb] < nz (a face
@@ -562,12 +569,16 @@ k_flux_fixup(DevGeom G, const double* __restrict__ prims, double* __restrict__ r
     int s = start[lb];
     if (s >= nz) return;
     if (s < 0) s = 0;
+    __shared__ DevGeom sG;                // see k_flux_div_generic
+    __shared__ spd_flux_desc sF;
+    if (threadIdx.x == 0) { sG = G; sF = F; }
+    __syncthreads();
     const int per_plane = G.nc[0]*G.nc[1];
     const int n = per_plane*(nz - s);
     for (int c = threadIdx.x; c < n; c += blockDim.x)
     {
         const int i = c % G.nc[0], j = (c / G.nc[0]) % G.nc[1], k = s + c/per_plane;
-        generic_cell(G, prims, rhs, F, increment, i, j, k, lb);
+        generic_cell(sG, prims, rhs, sF, increment, i, j, k, lb);
     }
 }
 
