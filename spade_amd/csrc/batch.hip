@@ -189,26 +189,25 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             return CellF{t[p], t[HPB + p], s0[p], s0[HPB + p], s0[2*HPB + p], t[2*HPB + p], t[3*HPB + p]};
         };
 
-        // stage inputs of the plane to be written next, requested before the
-        // primitive prefetch so that waiting for them never waits for it
-        double uin[5], u0[5];
-        auto load_stage_inputs = [&](int k)
+        // u0 of the plane to be written next (stages 2 and 3), requested before
+        // the state prefetch so that waiting for it never waits for the prefetch
+        double u0[5];
+        auto load_u0 = [&](int k)
         {
             const long long o = (long long)slot*5*4096 + (k*16 + j)*16 + i;
             #pragma unroll
-            for (int n = 0; n < 5; ++n)
-            {
-                uin[n] = Qin[o + n*4096];
-                u0[n] = HAS_U0 ? Q0[o + n*4096] : 0.0;
-            }
+            for (int n = 0; n < 5; ++n) u0[n] = HAS_U0 ? Q0[o + n*4096] : 0.0;
         };
-        load_stage_inputs(0);
+        load_u0(0);
 
-        // prologue: planes -1, 0, 1 resident, planes 2 .. 1 + PFB in flight
+        // prologue: planes -1, 0, 1 resident, planes 2 .. 1 + PFB in flight.
+        // The own column's conserved state of planes k and k+1 stays in
+        // registers (the stage input u_in); velocities of planes k, k+1 are in LDS.
         const PrimD qprev = to_prim(P, load(i, j, -1));
-        PrimD qcur = to_prim(P, load(i, j, 0));
-        PrimD qnxt = to_prim(P, load(i, j, 1));
-        CellF prev = to_f32(P, qprev), cur = to_f32(P, qcur), nxt = to_f32(P, qnxt);
+        Cons rcur = load(i, j, 0);
+        Cons rnxt = load(i, j, 1);
+        const PrimD qcur0 = to_prim(P, rcur), qnxt0 = to_prim(P, rnxt);
+        CellF prev = to_f32(P, qprev), cur = to_f32(P, qcur0), nxt = to_f32(P, qnxt0);
         Cons nraw[PFB], hraw[PFB];
         #pragma unroll
         for (int d = 0; d < PFB; ++d) { nraw[d] = load(i, j, 2 + d); hraw[d] = Cons{}; }
@@ -216,7 +215,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             float* s = sU[slotk(-1)];
             s[pown] = prev.u; s[HPB + pown] = prev.v; s[2*HPB + pown] = prev.w;
         }
-        put(0, pown, qcur, cur); put(1, pown, qnxt, nxt);
+        put(0, pown, qcur0, cur); put(1, pown, qnxt0, nxt);
         if (has_halo)
         {
             const CellF hp = to_f32(P, to_prim(P, load(hi, hj, -1)));
@@ -292,7 +291,8 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             const double* vk = sV[k & 1];
             const double dux = vk[pown + 1] - vk[pown - 1];                       // u(i+1) - u(i-1)
             const double dvy = vk[HPB + pown + HXB] - vk[HPB + pown - HXB];        // v(j+1) - v(j-1)
-            const double dwz = qnxt.w - wprev_w;                                   // w(k+1) - w(k-1)
+            const double* vn = sV[(k + 1) & 1];
+            const double dwz = vn[2*HPB + pown] - wprev_w;                         // w(k+1) - w(k-1)
             const double DU = -0.5*(dux*P.ix + dvy*P.iy + dwz*P.iz);               // -div u
             if (want_diag)
             {
@@ -300,11 +300,12 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 const double duy = vk[pown + HXB] - vk[pown - HXB];
                 const double dvx = vk[HPB + pown + 1] - vk[HPB + pown - 1];
                 const double dwx = vk[2*HPB + pown + 1] - vk[2*HPB + pown - 1];
-                const double dvz = qnxt.v - wprev_v, duz = qnxt.u - wprev_u;
+                const double dvz = vn[HPB + pown] - wprev_v, duz = vn[pown] - wprev_u;
                 const double ox = 0.5*(dwy*P.iy - dvz*P.iz);
                 const double oy = 0.5*(duz*P.iz - dwx*P.ix);
                 const double oz = 0.5*(dvx*P.ix - duy*P.iy);
-                ke_acc += 0.5*qcur.rho*(qcur.u*qcur.u + qcur.v*qcur.v + qcur.w*qcur.w);
+                const double uc = vk[pown], vc = vk[HPB + pown], wc = vk[2*HPB + pown];
+                ke_acc += 0.5*rcur.r*(uc*uc + vc*vc + wc*wc);
                 en_acc += 0.5*(ox*ox + oy*oy + oz*oz);
             }
             lds_barrier();
@@ -326,19 +327,20 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             #pragma unroll
             for (int n = 0; n < 5; ++n)
             {
-                const double stage = uin[n] + P.dt*res[n];
+                const double uin = n == 0 ? rcur.r : n == 1 ? rcur.E : n == 2 ? rcur.mu : n == 3 ? rcur.mv : rcur.mw;
+                const double stage = uin + P.dt*res[n];
                 Qout[o + n*4096] = HAS_U0 ? alpha*u0[n] + beta*stage : beta*stage;
             }
 
             // advance the window
-            wprev_u = qcur.u; wprev_v = qcur.v; wprev_w = qcur.w;
+            wprev_u = vk[pown]; wprev_v = vk[HPB + pown]; wprev_w = vk[2*HPB + pown];
             if (k + 1 < 16)
             {
-                load_stage_inputs(k + 1);
-                qcur = qnxt; cur = nxt;
-                qnxt = to_prim(P, nraw[0]);
-                nxt = to_f32(P, qnxt);
-                put(k + 2, pown, qnxt, nxt);
+                load_u0(k + 1);
+                rcur = rnxt; rnxt = nraw[0]; cur = nxt;
+                const PrimD q2 = to_prim(P, rnxt);
+                nxt = to_f32(P, q2);
+                put(k + 2, pown, q2, nxt);
                 #pragma unroll
                 for (int d = 0; d + 1 < PFB; ++d) nraw[d] = nraw[d + 1];
                 if (k + 2 + PFB <= 16) nraw[PFB - 1] = load(i, j, k + 2 + PFB);
@@ -378,11 +380,16 @@ __global__ void k_step_count(int* step) { if (threadIdx.x == 0 && blockIdx.x == 
 // Initial conditions for the blocks of one case (TGV or isentropic vortex), in
 // conserved FP64 variables.
 __global__ void __launch_bounds__(256)
-k_init_case(BatchDev B, const int* __restrict__ slots, int nslots, CaseParams P, CaseIC ic, double* __restrict__ Q)
+k_init_case(const int* __restrict__ stage, int nslots, CaseParams P, CaseIC ic, double* __restrict__ Q,
+            int* __restrict__ nbr, int* __restrict__ slot_case)
 {
+    // stage: per block of the case, {slot, case table, 27 neighbour slots}
     const int b = blockIdx.y;
     if (b >= nslots) return;
-    const int slot = slots[b];
+    const int* e = stage + 29*b;
+    const int slot = e[0];
+    if (blockIdx.x == 0 && threadIdx.x < 27) nbr[27*(long long)slot + threadIdx.x] = e[2 + threadIdx.x];
+    if (blockIdx.x == 0 && threadIdx.x == 27) slot_case[slot] = e[1];
     const int bx = b % ic.nb[0], by = (b / ic.nb[0]) % ic.nb[1], bz = b / (ic.nb[0]*ic.nb[1]);
     for (int c = threadIdx.x + blockIdx.x*blockDim.x; c < 4096; c += blockDim.x*gridDim.x)
     {
@@ -447,10 +454,11 @@ void launch_batch_step_count(const BatchDev& B, hipStream_t s)
     SPD_LAUNCH_CHECK();
 }
 
-void launch_batch_init(const BatchDev& B, const int* d_slots, int nslots, const CaseParams& P, const CaseIC& ic,
+void launch_batch_init(const BatchDev& B, const int* d_stage, int nslots, const CaseParams& P, const CaseIC& ic,
                        double* Q, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_init_case, dim3(4, nslots), dim3(256), 0, s, B, d_slots, nslots, P, ic, Q);
+    hipLaunchKernelGGL(k_init_case, dim3(4, nslots), dim3(256), 0, s, d_stage, nslots, P, ic, Q,
+                       const_cast<int*>(B.nbr), const_cast<int*>(B.slot_case));
     SPD_LAUNCH_CHECK();
 }
 

@@ -38,7 +38,7 @@ Batch::Batch(Ctx* c, long long budget_bytes, int law_in, int hist) : ctx(c), law
     SPD_HIP(hipMalloc((void**)&d_small, sizeof(int)*64));
     SPD_HIP(hipMalloc((void**)&d_cases, sizeof(CaseParams)*(size_t)max_cases));
     SPD_HIP(hipMalloc((void**)&d_diag, sizeof(double)*2*(size_t)hist_len*max_slots));
-    SPD_HIP(hipMalloc((void**)&d_init_slots, sizeof(int)*max_slots));
+    SPD_HIP(hipMalloc((void**)&d_init_slots, sizeof(int)*29*(size_t)max_slots));
     SPD_HIP(hipMemsetAsync(d_small, 0, sizeof(int)*64, ctx->stream));
     SPD_HIP(hipMemsetAsync(d_diag, 0, sizeof(double)*2*(size_t)hist_len*max_slots, ctx->stream));
     h_slot_case.assign(max_slots, -1);
@@ -163,17 +163,19 @@ int Batch::add_case(const spd_case_desc& d)
     }
     h_cases[c.table] = c.P;
     case_of_table[c.table] = c.id;
-    // tables and initial state (on the stream, before any later step)
+    // tables and initial state, in one staged copy (on the stream, ordered
+    // before any later step; pageable copies leave the host buffer free on return)
+    std::vector<int> stage(29*(size_t)nblk);
     for (long long b = 0; b < nblk; ++b)
     {
         const int s = c.slots[b];
-        SPD_HIP(hipMemcpyAsync(d_nbr + 27*(size_t)s, &h_nbr[27*(size_t)s], 27*sizeof(int), hipMemcpyHostToDevice, ctx->stream));
-        SPD_HIP(hipMemcpyAsync(d_slot_case + s, &h_slot_case[s], sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+        stage[29*b] = s;
+        stage[29*b + 1] = h_slot_case[s];
+        for (int e = 0; e < 27; ++e) stage[29*b + 2 + e] = h_nbr[27*(size_t)s + e];
     }
     SPD_HIP(hipMemcpyAsync(d_cases + c.table, &h_cases[c.table], sizeof(CaseParams), hipMemcpyHostToDevice, ctx->stream));
-    SPD_HIP(hipMemcpyAsync(d_init_slots, c.slots.data(), sizeof(int)*nblk, hipMemcpyHostToDevice, ctx->stream));
+    SPD_HIP(hipMemcpyAsync(d_init_slots, stage.data(), sizeof(int)*stage.size(), hipMemcpyHostToDevice, ctx->stream));
     launch_batch_init(dev, d_init_slots, (int)nblk, c.P, c.ic, Q[0], ctx->stream);
-    SPD_HIP(hipStreamSynchronize(ctx->stream));   // d_init_slots is reused by the next add
     c.steps_done = 0;
     cases.emplace(c.id, std::move(c));
     worklist_dirty = true;
