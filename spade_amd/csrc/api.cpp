@@ -209,6 +209,8 @@ int spd_grid_create(spd_ctx_t ctx, const spd_grid_desc* d, spd_grid_t* out)
         }
         require(d->num_exch >= 1 && d->num_exch <= 4, "num_exch must be in 1..4");
         for (int a = 0; a < 3; ++a) require(d->cells_in_block[a] >= d->num_exch, "blocks must be at least num_exch cells wide");
+        // per-block kernels put the block index in the second grid dimension
+        require((long long)d->num_blocks[0]*d->num_blocks[1]*d->num_blocks[2] <= 65535, "at most 65535 blocks per grid");
         auto* G = new spd_grid_s;
         std::unique_ptr<spd_grid_s> guard(G);
         Grid& g = G->g;

@@ -15,14 +15,18 @@ struct CellIt
     int i, j, k, lb;
 };
 
+// grid: (cells of a block / 256 rounded up, blocks); 32-bit index arithmetic
 __device__ inline bool interior_cell(const DevGeom& G, CellIt& c)
 {
-    const long long per_blk = (long long)G.nc[0]*G.nc[1]*G.nc[2];
-    const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-    if (t >= per_blk*G.nblk) return false;
-    c.lb = (int)(t / per_blk);
-    const int r = (int)(t % per_blk);
-    c.i = r % G.nc[0]; c.j = (r / G.nc[0]) % G.nc[1]; c.k = r / (G.nc[0]*G.nc[1]);
+    const unsigned per_blk = (unsigned)(G.nc[0]*G.nc[1]*G.nc[2]);
+    const unsigned r = blockIdx.x*blockDim.x + threadIdx.x;
+    if (r >= per_blk) return false;
+    c.lb = blockIdx.y;
+    const unsigned n0 = (unsigned)G.nc[0], n01 = (unsigned)(G.nc[0]*G.nc[1]);
+    c.k = (int)(r / n01);
+    const unsigned rr = r - (unsigned)c.k*n01;
+    c.j = (int)(rr / n0);
+    c.i = (int)(rr - (unsigned)c.j*n0);
     return true;
 }
 
@@ -56,9 +60,10 @@ k_rk_transform(DevGeom G, double* __restrict__ q, Coefs C, double gamma, double 
     long long idx[5];
     for (int v = 0; v < 5; ++v) { idx[v] = cell_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
     p2c(gamma, R, qq, cons);
-    for (int s = 0; s < C.n; ++s)
+    #pragma unroll
+    for (int s = 0; s < 3; ++s)               // static indices: C stays in scalar registers
     {
-        if (C.c[s] == 0.0) continue;
+        if (s >= C.n || C.c[s] == 0.0) continue;
         double inc[5];
         for (int v = 0; v < 5; ++v) inc[v] = C.c[s]*C.r[s][idx[v]];
         for (int v = 0; v < 5; ++v) cons[v] += inc[v];
@@ -111,10 +116,10 @@ k_rk_opt_s2(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, co
     for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
 }
 
-inline unsigned interior_blocks(const DevGeom& G)
+inline dim3 interior_blocks(const DevGeom& G)
 {
-    const long long n = (long long)G.nc[0]*G.nc[1]*G.nc[2]*G.nblk;
-    return (unsigned)((n + 255)/256);
+    const int n = G.nc[0]*G.nc[1]*G.nc[2];
+    return dim3((unsigned)((n + 255)/256), (unsigned)G.nblk);
 }
 
 } // namespace
@@ -125,25 +130,25 @@ void launch_rk_transform_stage(const Grid& g, double* q, const double* const* r,
     Coefs C{};
     C.n = nr;
     for (int j = 0; j < nr; ++j) { C.r[j] = r[j]; C.c[j] = coef[j]; }
-    hipLaunchKernelGGL(k_rk_transform, dim3(interior_blocks(g.geom)), dim3(256), 0, s, g.geom, q, C, gamma, R);
+    hipLaunchKernelGGL(k_rk_transform, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, C, gamma, R);
     SPD_LAUNCH_CHECK();
 }
 
 void launch_rk_opt_s0(const Grid& g, double* q, const double* r0, double dt, double gamma, double R, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rk_opt_s0, dim3(interior_blocks(g.geom)), dim3(256), 0, s, g.geom, q, r0, dt, gamma, R);
+    hipLaunchKernelGGL(k_rk_opt_s0, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, r0, dt, gamma, R);
     SPD_LAUNCH_CHECK();
 }
 
 void launch_rk_opt_s1(const Grid& g, double* q, double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rk_opt_s1, dim3(interior_blocks(g.geom)), dim3(256), 0, s, g.geom, q, r0, r1, dt, gamma, R);
+    hipLaunchKernelGGL(k_rk_opt_s1, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, r0, r1, dt, gamma, R);
     SPD_LAUNCH_CHECK();
 }
 
 void launch_rk_opt_s2(const Grid& g, double* q, const double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rk_opt_s2, dim3(interior_blocks(g.geom)), dim3(256), 0, s, g.geom, q, r0, r1, dt, gamma, R);
+    hipLaunchKernelGGL(k_rk_opt_s2, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, r0, r1, dt, gamma, R);
     SPD_LAUNCH_CHECK();
 }
 
