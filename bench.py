@@ -158,7 +158,6 @@ def main():
     host = np.zeros(g.host_shape)
     host[:, ng:-ng, ng:-ng, ng:-ng] = tgv_interior(nb, nc, L, p0)
     q = sa.GridArray(g).upload(host)
-    del host
     sa.create_exchange(q).exchange(q)
     rhs = sa.GridArray(g, fill=0.0)
     gas = sa.IdealGas(gamma, R)
@@ -198,6 +197,17 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
     }
+    # drop-in use from host memory: the reference-layout host arrays cross the
+    # boundary (layout conversion + PCIe) around one flux_div
+    t0 = time.perf_counter()
+    q.upload(host)
+    sa.flux_div(q, rhs, flux, increment=False)
+    out = rhs.download()
+    pcie_s = time.perf_counter() - t0
+    del host, out
+    line["host_arrays_inclusive"] = {"ms": round(pcie_s * 1e3, 1), "mcell_rhs_per_s": round(cells / pcie_s * 1e-6, 1),
+                                     "note": "upload of the primitive array + flux_div + download of the residual, "
+                                             "reference (AoS) layout on the host"}
     if args.copy_check:
         for _ in range(args.copy_check):
             rhs.copy_from(q)
