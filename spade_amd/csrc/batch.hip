@@ -329,7 +329,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             {
                 const double uin = n == 0 ? rcur.r : n == 1 ? rcur.E : n == 2 ? rcur.mu : n == 3 ? rcur.mv : rcur.mw;
                 const double stage = uin + P.dt*res[n];
-                Qout[o + n*4096] = HAS_U0 ? alpha*u0[n] + beta*stage : beta*stage;
+                __builtin_nontemporal_store(HAS_U0 ? alpha*u0[n] + beta*stage : beta*stage, Qout + o + n*4096);
             }
 
             // advance the window

@@ -481,7 +481,7 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             r -= sFy[n][(j + 1)*16 + i];
             r += zlo[n];
             r -= zhi[n];
-            ro[n*sv] = r;
+            __builtin_nontemporal_store(r, ro + n*sv);    // streamed: nothing in this launch reads it again
             zlo[n] = zhi[n];
         }
 
