@@ -77,25 +77,19 @@ void validate_flux(const spd_flux_desc* f, int ng)
     }
 }
 
-void host_to_device_layout(const Grid& g, const double* host, std::vector<double>& soa)
+// Host arrays in the reference layout (AoS, variable fastest) cross the
+// boundary through a device staging buffer of a few blocks; the layout is
+// converted by a kernel, so the host only copies contiguous bytes.
+double* ensure_stage(Grid& g)
 {
-    const DevGeom& G = g.geom;
-    soa.assign((size_t)G.sb*G.nblk, 0.0);
-    const long long cells = (long long)G.N[0]*G.N[1]*G.N[2];
-    for (int lb = 0; lb < G.nblk; ++lb)
-        for (long long c = 0; c < cells; ++c)
-            for (int v = 0; v < 5; ++v)
-                soa[(size_t)(lb*G.sb + v*G.sv + c)] = host[(size_t)5*(lb*cells + c) + v];
-}
-
-void device_to_host_layout(const Grid& g, const std::vector<double>& soa, double* host)
-{
-    const DevGeom& G = g.geom;
-    const long long cells = (long long)G.N[0]*G.N[1]*G.N[2];
-    for (int lb = 0; lb < G.nblk; ++lb)
-        for (long long c = 0; c < cells; ++c)
-            for (int v = 0; v < 5; ++v)
-                host[(size_t)5*(lb*cells + c) + v] = soa[(size_t)(lb*G.sb + v*G.sv + c)];
+    if (!g.d_stage)
+    {
+        const long long per_block = g.geom.sb;                     // doubles of one block (>= 5 stored cells x 5)
+        const long long want = (64LL << 20)/(8*per_block);        // about 64 MiB of staging
+        g.stage_blocks = (int)std::max(1LL, std::min<long long>(want, g.geom.nblk));
+        SPD_HIP(hipMalloc((void**)&g.d_stage, sizeof(double)*per_block*g.stage_blocks));
+    }
+    return g.d_stage;
 }
 
 void apply_bc(spd_integrator_s* it)
@@ -296,6 +290,7 @@ int spd_grid_destroy(spd_grid_t grid)
         SPD_HIP(hipFree(grid->g.d_tables));
         SPD_HIP(hipFree(grid->g.d_nbr));
         SPD_HIP(hipFree(grid->g.d_start));
+        if (grid->g.d_stage) SPD_HIP(hipFree(grid->g.d_stage));
         delete grid;
     });
 }
@@ -329,10 +324,16 @@ int spd_array_upload(spd_array_t arr, const double* host)
     return guarded([&]
     {
         require(arr && host, "null argument");
-        const Grid& g = *arr->a.grid;
-        std::vector<double> soa;
-        host_to_device_layout(g, host, soa);
-        SPD_HIP(hipMemcpyAsync(arr->a.d, soa.data(), soa.size()*sizeof(double), hipMemcpyHostToDevice, g.ctx->stream));
+        Grid& g = *arr->a.grid;
+        double* stage = ensure_stage(g);
+        const long long cells = (long long)g.geom.N[0]*g.geom.N[1]*g.geom.N[2];
+        for (int lb0 = 0; lb0 < g.geom.nblk; lb0 += g.stage_blocks)
+        {
+            const int nb = std::min(g.stage_blocks, g.geom.nblk - lb0);
+            SPD_HIP(hipMemcpyAsync(stage, host + 5*cells*lb0, sizeof(double)*5*cells*nb, hipMemcpyHostToDevice,
+                                   g.ctx->stream));
+            launch_aos_to_soa(g, stage, arr->a.d, lb0, nb, g.ctx->stream);
+        }
         SPD_HIP(hipStreamSynchronize(g.ctx->stream));
     });
 }
@@ -342,11 +343,17 @@ int spd_array_download(spd_array_t arr, double* host)
     return guarded([&]
     {
         require(arr && host, "null argument");
-        const Grid& g = *arr->a.grid;
-        std::vector<double> soa((size_t)g.geom.sb*g.geom.nblk);
-        SPD_HIP(hipMemcpyAsync(soa.data(), arr->a.d, soa.size()*sizeof(double), hipMemcpyDeviceToHost, g.ctx->stream));
+        Grid& g = *arr->a.grid;
+        double* stage = ensure_stage(g);
+        const long long cells = (long long)g.geom.N[0]*g.geom.N[1]*g.geom.N[2];
+        for (int lb0 = 0; lb0 < g.geom.nblk; lb0 += g.stage_blocks)
+        {
+            const int nb = std::min(g.stage_blocks, g.geom.nblk - lb0);
+            launch_soa_to_aos(g, arr->a.d, stage, lb0, nb, g.ctx->stream);
+            SPD_HIP(hipMemcpyAsync(host + 5*cells*lb0, stage, sizeof(double)*5*cells*nb, hipMemcpyDeviceToHost,
+                                   g.ctx->stream));
+        }
         SPD_HIP(hipStreamSynchronize(g.ctx->stream));
-        device_to_host_layout(g, soa, host);
     });
 }
 

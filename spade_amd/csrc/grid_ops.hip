@@ -62,6 +62,35 @@ k_wall_fill(DevGeom G, double* __restrict__ a, int idir, int pm, double t_wall)
     a[cell_index(G, lb, 4, idx[0], idx[1], idx[2])] = -w;
 }
 
+// Layout conversion on the device for nb blocks starting at block lb0:
+// stage holds them in the reference's host layout (AoS, variable fastest,
+// stored cells including ghosts, mem_map.h:484-496), a is the device array.
+__global__ void __launch_bounds__(256)
+k_aos_to_soa(DevGeom G, const double* __restrict__ stage, double* __restrict__ a, int lb0)
+{
+    const int cells = G.N[0]*G.N[1]*G.N[2];
+    const int c = blockIdx.x*blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const int b = blockIdx.y;
+    const double* e = stage + 5*((long long)b*cells + c);
+    double* d = a + (long long)(lb0 + b)*G.sb + c;
+    #pragma unroll
+    for (int v = 0; v < 5; ++v) d[v*G.sv] = e[v];
+}
+
+__global__ void __launch_bounds__(256)
+k_soa_to_aos(DevGeom G, const double* __restrict__ a, double* __restrict__ stage, int lb0)
+{
+    const int cells = G.N[0]*G.N[1]*G.N[2];
+    const int c = blockIdx.x*blockDim.x + threadIdx.x;
+    if (c >= cells) return;
+    const int b = blockIdx.y;
+    double* e = stage + 5*((long long)b*cells + c);
+    const double* d = a + (long long)(lb0 + b)*G.sb + c;
+    #pragma unroll
+    for (int v = 0; v < 5; ++v) e[v] = d[v*G.sv];
+}
+
 __global__ void __launch_bounds__(256)
 k_fill(double* __restrict__ a, long long n, double value)
 {
@@ -163,6 +192,22 @@ void launch_wall_fill(const Grid& g, double* a, int face, double t_wall, hipStre
     ext[idir] = G.ng;
     const int n = ext[0]*ext[1]*ext[2];
     hipLaunchKernelGGL(k_wall_fill, dim3((n + 255)/256, G.nblk), dim3(256), 0, s, G, a, idir, pm, t_wall);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_aos_to_soa(const Grid& g, const double* stage, double* a, int lb0, int nb, hipStream_t s)
+{
+    const DevGeom& G = g.geom;
+    const int cells = G.N[0]*G.N[1]*G.N[2];
+    hipLaunchKernelGGL(k_aos_to_soa, dim3((cells + 255)/256, nb), dim3(256), 0, s, G, stage, a, lb0);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_soa_to_aos(const Grid& g, const double* a, double* stage, int lb0, int nb, hipStream_t s)
+{
+    const DevGeom& G = g.geom;
+    const int cells = G.N[0]*G.N[1]*G.N[2];
+    hipLaunchKernelGGL(k_soa_to_aos, dim3((cells + 255)/256, nb), dim3(256), 0, s, G, a, stage, lb0);
     SPD_LAUNCH_CHECK();
 }
 

@@ -59,6 +59,8 @@ struct Grid
     double* d_tables = nullptr;   // dx, inv_dx, bmin
     int* d_nbr = nullptr;         // [26*lb + n] source block for ghost region n, -1 = not exchanged
     int* d_start = nullptr;       // [lb] first plane the LDS-staged flux kernel left to the fix-up kernel
+    double* d_stage = nullptr;    // host-layout staging for upload / download (stage_blocks blocks), lazily allocated
+    int stage_blocks = 0;
     std::vector<double> h_dx, h_inv_dx, h_bmin;
 };
 
@@ -110,6 +112,8 @@ bool flux_div_fast_eligible(const Grid& g, const spd_flux_desc& f);
 void launch_exchange(const Grid& g, double* a, hipStream_t s);
 void launch_wall_fill(const Grid& g, double* a, int face, double t_wall, hipStream_t s);
 void launch_fill(const Grid& g, double* a, double value, hipStream_t s);
+void launch_aos_to_soa(const Grid& g, const double* stage, double* a, int lb0, int nb, hipStream_t s);
+void launch_soa_to_aos(const Grid& g, const double* a, double* stage, int lb0, int nb, hipStream_t s);
 // RK stage: q <- c2p(p2c(q) + sum_j coef[j] * r_j) on interior cells
 void launch_rk_transform_stage(const Grid& g, double* q, const double* const* r, const double* coef, int nr,
                                double gamma, double R, hipStream_t s);
