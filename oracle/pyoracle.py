@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
 MAP_IDENTITY, MAP_TANH = 0, 1
-TERM_KEEP, TERM_TOTANI, TERM_PDISS, TERM_WENO, TERM_FWENO, TERM_FIRST, TERM_VISC = range(1, 8)
+TERM_KEEP, TERM_TOTANI, TERM_PDISS, TERM_WENO, TERM_FWENO, TERM_FIRST, TERM_VISC, TERM_HYBRID = range(1, 9)
 SENSOR_CONST, SENSOR_DUCROS = 0, 1
 SPLIT_RUSANOV, SPLIT_PHYS = 0, 1
 LAW_CONST, LAW_POWER = 0, 1
@@ -39,7 +39,7 @@ class Term(C.Structure):
                 ("sensor_param", C.c_double), ("eps_p", C.c_double), ("eps_T", C.c_double),
                 ("split", C.c_int), ("smooth", C.c_int), ("law", C.c_int),
                 ("mu", C.c_double), ("T_ref", C.c_double), ("power", C.c_double),
-                ("prandtl", C.c_double)]
+                ("prandtl", C.c_double), ("diss_kind", C.c_int), ("full", C.c_int)]
 
 
 class Flux(C.Structure):

@@ -256,6 +256,17 @@ int main(int argc, char** argv)
         run("fweno_lin", spade::convective::fweno_t<decltype(air), spade::convective::disable_smooth>(air));
         run("first_order", spade::convective::first_order_t<decltype(air)>(air));
         run("keep2_pdiss_visc_const", spade::omni::compose(spade::convective::cent_keep<2>(air), pd_ducros, vsc_const));
+        // central / WENO blends (navier-stokes/hybrid_scheme.h)
+        run("hybrid_keep2_weno_ducros_full",
+            spade::convective::hybrid_scheme_t(spade::convective::cent_keep<2>(air), spade::convective::weno_t<decltype(rus)>(rus),
+                                               s_ducros, spade::convective::full_flux));
+        run("hybrid_keep2_fweno_const_diss",
+            spade::convective::hybrid_scheme_t(spade::convective::cent_keep<2>(air), spade::convective::fweno_t<decltype(air)>(air),
+                                               s_const, spade::convective::diss_flux));
+        run("hybrid_keep4_weno_ducros_full_visc",
+            spade::omni::compose(spade::convective::hybrid_scheme_t(spade::convective::cent_keep<4>(air),
+                                                                    spade::convective::weno_t<decltype(rus)>(rus),
+                                                                    s_ducros, spade::convective::full_flux), vsc_const));
 
         // increment mode: rhs accumulates onto its existing content
         {

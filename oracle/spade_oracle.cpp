@@ -571,6 +571,29 @@ void f_visc(const Ctx& C, const Face& f, const ora_term& t, double out[5])
     out[4] = -(n[0]*tau[2][0] + n[1]*tau[2][1] + n[2]*tau[2][2]);
 }
 
+// hybrid_scheme_t (navier-stokes/hybrid_scheme.h:14-48): central cent_keep<order>
+// blended with a WENO flux by the blender's face sensor b:
+// full_flux: (1 - b) F0 + b F1; diss_flux: F0 + b F1 (coefficient 1 applied)
+void f_hybrid(const Ctx& C, const Face& f, const ora_term& t, double out[5])
+{
+    double blend = t.sensor_param;
+    if (t.sensor == ORA_SENSOR_DUCROS)
+    {
+        double gr[3][5]; face_grad(C.G, C.a, f, gr);
+        blend = ducros(gr, t.sensor_param);
+    }
+    double f0[5], f1[5];
+    f_keep(C, f, t.order, f0);
+    if (t.diss_kind == ORA_TERM_FWENO) f_fweno(C, f, t, f1);
+    else                               f_weno(C, f, t, f1);
+    double coeff0 = 1.0 - blend;
+    const double coeff1 = blend;
+    if (!t.full) coeff0 = 1.0;
+    for (int n = 0; n < 5; ++n) f0[n] *= coeff0;
+    for (int n = 0; n < 5; ++n) f1[n] *= coeff1;
+    for (int n = 0; n < 5; ++n) out[n] = f0[n] + f1[n];
+}
+
 void eval_term(const Ctx& C, const Face& f, const ora_term& t, double out[5])
 {
     switch (t.kind)
@@ -582,6 +605,7 @@ void eval_term(const Ctx& C, const Face& f, const ora_term& t, double out[5])
         case ORA_TERM_FWENO:  f_fweno(C, f, t, out); break;
         case ORA_TERM_FIRST:  f_first(C, f, out); break;
         case ORA_TERM_VISC:   f_visc(C, f, t, out); break;
+        case ORA_TERM_HYBRID: f_hybrid(C, f, t, out); break;
         default: for (int n = 0; n < 5; ++n) out[n] = 0.0;
     }
 }

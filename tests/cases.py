@@ -30,11 +30,26 @@ FLUX_CASES = {
     "fweno_lin": [dict(kind=5, smooth=0)],
     "first_order": [dict(kind=6)],
     "keep2_pdiss_visc_const": [K2, PDD, VC],
+    # hybrid_scheme_t: cent_keep<order> blended with WENO by a face sensor
+    "hybrid_keep2_weno_ducros_full": [dict(kind=8, order=2, sensor=1, sensor_param=1e-3, split=0, smooth=1,
+                                           diss_kind=4, full=1)],
+    "hybrid_keep2_fweno_const_diss": [dict(kind=8, order=2, sensor=0, sensor_param=0.7, smooth=1,
+                                           diss_kind=5, full=0)],
+    "hybrid_keep4_weno_ducros_full_visc": [dict(kind=8, order=4, sensor=1, sensor_param=1e-3, split=0, smooth=1,
+                                                diss_kind=4, full=1), VC],
 }
 
 # cases whose arithmetic involves pow(): equal to the reference within
 # round-off, not bit for bit, on the GPU (device pow vs glibc pow)
 USES_POW = {"visc_pow", "keep2_visc_pow"}
+
+# compositions the CPU oracle reproduces bit for bit against the reference but
+# the GPU library does not offer: its first hybrid_scheme_t kernel build was
+# followed by a GPU memory fault (in an unrelated launch) whose cause is not
+# established, so the GPU side was taken out and these cases are skipped there
+GPU_UNSUPPORTED = {k for k in FLUX_CASES if k.startswith("hybrid")}
+GPU_UNSUPPORTED_REASON = ("hybrid_scheme_t is pinned on the CPU oracle only; its GPU path was removed after a "
+                          "memory fault of unestablished cause")
 
 G1 = dict(nb=(2, 3, 2), nc=(4, 8, 4), ng=2)
 G2 = dict(nb=(2, 2, 2), nc=(8, 8, 8), ng=2)

@@ -76,9 +76,11 @@ def test_channel_step_matches_oracle():
     it = sa.Integrator(q, flux, c["dt"], scheme=sa.ssprk3, walls=(2, 3), t_wall=c["T_wall"])
     it.advance(1)
     got = gi.interior(q.download(), c["ng"])
+    print("channel: GPU step done, oracle step on the host", flush=True)
     ref = start
     bc = po.BC(wall_mask=(1 << 2) | (1 << 3), t_wall=c["T_wall"])
     po.ssprk3_step(og, po.Gas(c["gamma"], c["R"]), oflux, bc, c["dt"], ref)
+    print("channel: oracle step done", flush=True)
     err = gi.rel_maxnorm(got, gi.interior(ref, c["ng"]))
     print("channel 1 step rel err per var", err, flush=True)
     assert np.all(err <= TOL), err
@@ -95,7 +97,10 @@ def test_channel_200_steps(with_pdiss):
     it = sa.Integrator(q, flux, c["dt"], scheme=sa.ssprk3, walls=(2, 3), t_wall=c["T_wall"])
     # mass of the initial state (ghosts are not part of the sum)
     m0 = _mass(c["host"], c, c["R"])
-    it.advance(200)
+    for chunk in range(4):
+        it.advance(50)
+        g.ctx.sync()
+        print("channel: steps", 50 * (chunk + 1), flush=True)
     out = q.download()
     inner = gi.interior(out, c["ng"])
     assert np.all(np.isfinite(inner))

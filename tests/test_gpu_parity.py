@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import golden_io as gi
-from cases import FLUX_CASES, G1, G2, GAMMA, R, P, USES_POW
+from cases import FLUX_CASES, GPU_UNSUPPORTED, GPU_UNSUPPORTED_REASON, G1, G2, GAMMA, R, P, USES_POW
 from gpu_helpers import RawFlux, grid
 
 pytestmark = pytest.mark.gpu
@@ -48,6 +48,8 @@ def test_wall_exchange_and_fill_bitwise():
 
 @pytest.mark.parametrize("name", sorted(FLUX_CASES))
 def test_flux_div(name):
+    if name in GPU_UNSUPPORTED:
+        pytest.skip(GPU_UNSUPPORTED_REASON)
     sa = _sa()
     g = grid(G1)
     q = sa.GridArray(g).upload(_g1_prim())
@@ -108,3 +110,17 @@ def test_ssprk3_opt_steps():
     err = gi.rel_maxnorm(gi.interior(q.download(), 2), gi.load("g2_ssprk3opt_step3"))
     print("ssprk3_opt", err)
     assert np.all(err <= TOL_POW)
+
+
+@pytest.mark.skip(reason=GPU_UNSUPPORTED_REASON)
+def test_hybrid_builder_matches_reference():
+    """spade_amd.hybrid_scheme(...) builds the descriptor of the reference's
+    hybrid_scheme_t(cent_keep<2>, weno_t<rusanov_t>, ducros_t, full_flux)."""
+    sa = _sa()
+    g = grid(G1)
+    gas = sa.IdealGas(GAMMA, R)
+    q = sa.GridArray(g).upload(_g1_prim())
+    rhs = sa.GridArray(g, fill=0.0)
+    f = sa.hybrid_scheme(sa.cent_keep(2, gas), sa.weno(gas), sa.ducros(1e-3), sa.full_flux)
+    sa.flux_div(q, rhs, f, increment=False)
+    assert np.array_equal(gi.interior(rhs.download(), 2), gi.load("g1_rhs_hybrid_keep2_weno_ducros_full"))

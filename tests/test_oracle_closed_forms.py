@@ -29,6 +29,10 @@ ALL = {
     "keep2": [K2], "keep4": [K4], "totani": [dict(kind=po.TERM_TOTANI)],
     "keep2_pdiss_visc": [K2, PD, VP], "weno": [dict(kind=po.TERM_WENO, split=0, smooth=1)],
     "fweno": [dict(kind=po.TERM_FWENO, smooth=1)], "visc_const": [VC],
+    "hybrid_weno_ducros": [dict(kind=po.TERM_HYBRID, order=2, sensor=po.SENSOR_DUCROS, sensor_param=1e-3,
+                                split=0, smooth=1, diss_kind=po.TERM_WENO, full=1)],
+    "hybrid_fweno_const_diss": [dict(kind=po.TERM_HYBRID, order=4, sensor=po.SENSOR_CONST, sensor_param=0.3,
+                                     smooth=1, diss_kind=po.TERM_FWENO, full=0)],
 }
 TANH_Y = (po.MAP_TANH, -1.0, 1.0, 1.5)
 
