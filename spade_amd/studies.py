@@ -112,8 +112,12 @@ def run_ensemble(specs, steps, tune=True):
     b.ctx.sync()
     dt = time.perf_counter() - t0
     cells = sum(s.num_cells for s in specs)
+    # algorithmic bytes of one step per cell: stage 1 reads u0 and writes stage 1
+    # (80 B), stages 2 and 3 read their input and u0 and write (120 B each)
+    alg_bytes = cells * steps * (80.0 + 120.0 + 120.0)
     out = {"cases": len(specs), "blocks": blocks, "cells": cells, "steps": steps, "seconds": dt,
-           "cell_updates_per_s": cells * steps / dt, "tuned_shape": shape}
+           "cell_updates_per_s": cells * steps / dt, "mcell_rhs_per_s": 3.0 * cells * steps / dt * 1e-6,
+           "hbm_algorithmic_gb_s": alg_bytes / dt * 1e-9, "hbm_frac": alg_bytes / dt / 8e12, "tuned_shape": shape}
     b.close()
     del ids
     return out
