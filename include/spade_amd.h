@@ -23,7 +23,8 @@ enum {
     SPD_ERR_ARG = 1,          /* bad argument / size / configuration (reference: except::sp_exception, core/except.h:12-30) */
     SPD_ERR_HIP = 2,          /* HIP runtime or launch failure (reference: CUDA error check after every launch, dispatch/execute.h:85-96) */
     SPD_ERR_UNSUPPORTED = 3,  /* combination the library does not implement */
-    SPD_ERR_MEMORY = 4        /* device allocation failure (reference: mem_exception, dispatch/device_allocator.h:30-44) */
+    SPD_ERR_MEMORY = 4,       /* device allocation failure (reference: mem_exception, dispatch/device_allocator.h:30-44) */
+    SPD_ERR_IO = 5            /* a file could not be opened, read or written (the reference does not check its streams) */
 };
 
 typedef struct spd_ctx_s*        spd_ctx_t;
@@ -205,6 +206,32 @@ int spd_batch_tune(spd_batch_t b, int* grid, int* interleaved);
 int spd_batch_use_graph(spd_batch_t b, int enable);
 /* Launch shape in use (persistent workgroups, work order). */
 int spd_batch_set_shape(spd_batch_t b, int grid, int interleaved);
+
+/* ---- on-disk formats: spade::io (io/io_vtk.h, io/io_native.h) ---- */
+
+/* io::output_vtk(out_dir, basename, array) (io/io_vtk.h:276-288): writes
+ * <out_dir>/<basename>.pvts (a PStructuredGrid whose pieces stack the blocks
+ * along z, io_vtk.h:131-174) and one StructuredGrid file per block,
+ * <out_dir>/data_<basename>/b<9-digit block>.vts (io_vtk.h:176-273): interior
+ * cell data per variable (i fastest) and the (n+1)^3 node coordinates x =
+ * map(box min + i dx), Float64, inline binary as base64 of a 4-byte byte
+ * count followed by base64 of the data (core/base_64.h:63-69).  names: the 5
+ * variable names, NULL for the primitive names P,T,U,V,W (prim_t::name,
+ * navier-stokes/fluid_state.h:28-32).  Creates the two directories if
+ * missing. */
+int spd_io_output_vtk(spd_array_t arr, const char* out_dir, const char* basename, const char* const* names);
+/* The same output for an array image in host memory (reference layout, as
+ * spd_array_download writes it) on the grid desc describes: a CPU-resident
+ * grid_array in the reference.  Needs no device. */
+int spd_io_output_vtk_host(const spd_grid_desc* desc, const double* host, const char* out_dir, const char* basename,
+                           const char* const* names);
+/* io::binary_write(filename, array) (io/io_native.h:19-45): every block's
+ * stored cells, ghosts included, at byte offset lb * block_bytes, i.e. the
+ * array's host image in the reference layout. */
+int spd_io_binary_write(spd_array_t arr, const char* filename);
+/* io::binary_read(filename, array) (io/io_native.h:47-55); fails with
+ * SPD_ERR_IO unless the file holds exactly the array's host image size. */
+int spd_io_binary_read(spd_array_t arr, const char* filename);
 
 #ifdef __cplusplus
 }

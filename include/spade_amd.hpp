@@ -201,4 +201,29 @@ inline double max_wavespeed(const grid_array& q, const ideal_gas_t& gas)
     return m;
 }
 
+// spade::io (io/io_vtk.h:276-288, io/io_native.h:39-55)
+namespace io
+{
+    // names: the 5 variable names; empty for P,T,U,V,W (prim_t)
+    inline void output_vtk(const std::string& out_dir, const std::string& basename, const grid_array& arr,
+                           const std::vector<std::string>& names = {})
+    {
+        const char* n[5];
+        if (!names.empty())
+        {
+            if (names.size() != 5) throw error(SPD_ERR_ARG, "output_vtk needs 5 variable names");
+            for (int v = 0; v < 5; ++v) n[v] = names[v].c_str();
+        }
+        check(spd_io_output_vtk(arr.handle(), out_dir.c_str(), basename.c_str(), names.empty() ? nullptr : n));
+    }
+    inline void binary_write(const std::string& filename, const grid_array& arr)
+    {
+        check(spd_io_binary_write(arr.handle(), filename.c_str()));
+    }
+    inline void binary_read(const std::string& filename, grid_array& arr)
+    {
+        check(spd_io_binary_read(arr.handle(), filename.c_str()));
+    }
+}
+
 } // namespace spade_amd

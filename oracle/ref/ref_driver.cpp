@@ -223,6 +223,9 @@ int main(int argc, char** argv)
         handle.exchange(prim);
         dump("g1_prim", prim);
         dump("g1_prim_ghost", prim, true);
+        // the reference's parallel VTK StructuredGrid output of the same array
+        // (io/io_vtk.h:276-288): g1.pvts plus data_g1/b<lb>.vts
+        spade::io::output_vtk(g_outdir + "/vtk_g1", "g1", prim);
 
         spade::viscous_laws::constant_viscosity_t<real_t> visc_const(mu_ref, prandtl);
         power_law_all_t<real_t> visc_pow(mu_ref, T0, power, prandtl);

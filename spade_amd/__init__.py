@@ -20,6 +20,8 @@ ABI of libspade_amd.so (include/spade_amd.h):
   algs::boundary_fill (wall mirror)              boundary_fill_wall
   integrator_t(...).advance() with ssprk3_t /    Integrator(q, flux, bc, scheme, dt).advance()
   ssprk3_opt and state_transform_t
+  io::output_vtk(dir, name, arr) (io/io_vtk.h)   output_vtk(dir, name, arr[, names])
+  io::binary_write / binary_read (io_native.h)   binary_write(file, arr) / binary_read(file, arr)
 
 Every array lives on the GPU; there is no CPU fallback.  If the library is
 missing, importing works but the first call raises FileNotFoundError.
@@ -36,6 +38,8 @@ __all__ = [
     "weno", "fweno", "first_order", "visc_lr", "constant_viscosity", "power_law", "const_sensor",
     "ducros", "compose", "flux_div", "create_exchange", "boundary_fill_wall", "Integrator",
     "ssprk3", "ssprk3_opt", "max_wavespeed", "tanh_map", "SpadeError", "diagnostics", "CaseSpec", "Batch",
+    "grid_desc", "output_vtk", "output_vtk_host", "binary_write", "binary_read", "PRIM_NAMES", "CONS_NAMES",
+    "FLUX_NAMES",
 ]
 
 # ---------------------------------------------------------------- context ---
@@ -94,6 +98,25 @@ def tanh_map(p0, p1, beta):
     return (_abi.MAP_TANH, float(p0), float(p1), float(beta))
 
 
+def grid_desc(cells_in_block, num_blocks, bounds, num_exch=2, periodic=(True, True, True),
+              maps=(None, None, None)):
+    """The C ABI's grid description (spd_grid_desc) of a CartesianGrid with these
+    arguments; needs no device."""
+    d = _abi.GridDesc()
+    for a in range(3):
+        d.num_blocks[a] = int(num_blocks[a])
+        d.cells_in_block[a] = int(cells_in_block[a])
+        d.lo[a], d.hi[a] = float(bounds[a][0]), float(bounds[a][1])
+        d.periodic[a] = int(bool(periodic[a]))
+        m = maps[a]
+        if m is None:
+            d.map[a].kind = _abi.MAP_IDENTITY
+        else:
+            d.map[a].kind, d.map[a].p0, d.map[a].p1, d.map[a].beta = m
+    d.num_exch = int(num_exch)
+    return d
+
+
 class CartesianGrid:
     """Uniform multi-block grid (cartesian_blocks_t + cartesian_grid_t).
 
@@ -104,18 +127,7 @@ class CartesianGrid:
     def __init__(self, cells_in_block, num_blocks, bounds, num_exch=2, periodic=(True, True, True),
                  maps=(None, None, None), ctx=None):
         self.ctx = ctx or default_context()
-        d = _abi.GridDesc()
-        for a in range(3):
-            d.num_blocks[a] = int(num_blocks[a])
-            d.cells_in_block[a] = int(cells_in_block[a])
-            d.lo[a], d.hi[a] = float(bounds[a][0]), float(bounds[a][1])
-            d.periodic[a] = int(bool(periodic[a]))
-            m = maps[a]
-            if m is None:
-                d.map[a].kind = _abi.MAP_IDENTITY
-            else:
-                d.map[a].kind, d.map[a].p0, d.map[a].p1, d.map[a].beta = m
-        d.num_exch = int(num_exch)
+        d = grid_desc(cells_in_block, num_blocks, bounds, num_exch, periodic, maps)
         self.desc = d
         self._h = C.c_void_p()
         check(load().spd_grid_create(self.ctx._h, C.byref(d), C.byref(self._h)))
@@ -485,3 +497,54 @@ def max_wavespeed(prims, gas):
     out = C.c_double()
     check(load().spd_max_wavespeed(prims._h, gas.gamma, gas.R, C.byref(out)))
     return out.value
+
+
+# --------------------------------------------------------------------- io ---
+
+# variable names of the reference's state types (navier-stokes/fluid_state.h:28-75)
+PRIM_NAMES = ("P", "T", "U", "V", "W")
+CONS_NAMES = ("rho", "rhoH", "rhoU", "rhoV", "rhoW")
+FLUX_NAMES = ("continuity", "energy", "x_momentum", "y_momentum", "z_momentum")
+
+
+def _names(names):
+    if names is None:
+        return None
+    names = [str(n) for n in names]
+    if len(names) != 5:
+        raise ValueError("output_vtk needs 5 variable names")
+    arr = (C.c_char_p * 5)(*[n.encode() for n in names])
+    return C.cast(arr, C.POINTER(C.c_char_p)), arr
+
+
+def output_vtk(out_dir, basename, arr, names=None):
+    """io::output_vtk(out_dir, basename, array) (io/io_vtk.h:276-288):
+    <out_dir>/<basename>.pvts + <out_dir>/data_<basename>/b<lb>.vts.
+    names: the 5 variable names (default PRIM_NAMES, the prim_t names)."""
+    n = _names(names)
+    check(load().spd_io_output_vtk(arr._h, str(out_dir).encode(), str(basename).encode(), n[0] if n else None))
+
+
+def output_vtk_host(out_dir, basename, desc, host, names=None):
+    """output_vtk of an array image in host memory (reference layout, shape
+    (blocks, Nz, Ny, Nx, 5)) on the grid `desc` (grid_desc(...)); needs no device."""
+    a = np.ascontiguousarray(host, dtype=np.float64)
+    ng = desc.num_exch
+    nb = desc.num_blocks[0] * desc.num_blocks[1] * desc.num_blocks[2]
+    shape = (nb,) + tuple(desc.cells_in_block[a_] + 2 * ng for a_ in (2, 1, 0)) + (5,)
+    if a.shape != shape:
+        raise ValueError(f"host array shape {a.shape} != {shape}")
+    n = _names(names)
+    check(load().spd_io_output_vtk_host(C.byref(desc), a.ctypes.data_as(C.POINTER(C.c_double)),
+                                        str(out_dir).encode(), str(basename).encode(), n[0] if n else None))
+
+
+def binary_write(filename, arr):
+    """io::binary_write(filename, array) (io/io_native.h:39-45): the array's
+    host image (reference layout, ghosts included), block after block."""
+    check(load().spd_io_binary_write(arr._h, str(filename).encode()))
+
+
+def binary_read(filename, arr):
+    """io::binary_read(filename, array) (io/io_native.h:47-55)."""
+    check(load().spd_io_binary_read(arr._h, str(filename).encode()))

@@ -9,7 +9,7 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libspade_amd.so")
 
-SPD_OK, SPD_ERR_ARG, SPD_ERR_HIP, SPD_ERR_UNSUPPORTED, SPD_ERR_MEMORY = range(5)
+SPD_OK, SPD_ERR_ARG, SPD_ERR_HIP, SPD_ERR_UNSUPPORTED, SPD_ERR_MEMORY, SPD_ERR_IO = range(6)
 MAP_IDENTITY, MAP_TANH = 0, 1
 TERM_KEEP, TERM_TOTANI, TERM_PDISS, TERM_WENO, TERM_FWENO, TERM_FIRST, TERM_VISC = range(1, 8)
 SENSOR_CONST, SENSOR_DUCROS = 0, 1
@@ -100,6 +100,10 @@ SIGNATURES = {
     "spd_batch_tune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "spd_batch_use_graph": (C.c_int, [_vp, C.c_int]),
     "spd_batch_set_shape": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "spd_io_output_vtk": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p)]),
+    "spd_io_output_vtk_host": (C.c_int, [C.POINTER(GridDesc), _dp, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p)]),
+    "spd_io_binary_write": (C.c_int, [_vp, C.c_char_p]),
+    "spd_io_binary_read": (C.c_int, [_vp, C.c_char_p]),
 }
 
 _lib = None

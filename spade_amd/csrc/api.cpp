@@ -1,5 +1,5 @@
 // C ABI of libspade_amd (declared in include/spade_amd.h).
-#include "spd_internal.hpp"
+#include "api_internal.hpp"
 #include "batch_host.hpp"
 #include <cmath>
 #include <cstring>
@@ -15,9 +15,6 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 using namespace spd;
 
-struct spd_ctx_s        { Ctx c; };
-struct spd_grid_s       { Grid g; };
-struct spd_array_s      { Array a; };
 struct spd_batch_s { std::unique_ptr<Batch> b; };
 struct spd_integrator_s
 {
@@ -30,25 +27,6 @@ struct spd_integrator_s
 };
 
 namespace {
-
-template <typename F> int guarded(F&& fn)
-{
-    try
-    {
-        fn();
-        return SPD_OK;
-    }
-    catch (const ArgError& e) { set_error(e.msg); return SPD_ERR_ARG; }
-    catch (const HipError& e)
-    {
-        set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
-        return e.code == hipErrorOutOfMemory ? SPD_ERR_MEMORY : SPD_ERR_HIP;
-    }
-    catch (const std::bad_alloc&) { set_error("host allocation failed"); return SPD_ERR_MEMORY; }
-    catch (const std::exception& e) { set_error(e.what()); return SPD_ERR_ARG; }
-}
-
-inline void require(bool ok, const std::string& msg) { if (!ok) throw ArgError{msg}; }
 
 void validate_flux(const spd_flux_desc* f, int ng)
 {
@@ -191,20 +169,8 @@ int spd_grid_create(spd_ctx_t ctx, const spd_grid_desc* d, spd_grid_t* out)
 {
     return guarded([&]
     {
-        require(ctx && d && out, "null argument");
-        for (int a = 0; a < 3; ++a)
-        {
-            require(d->num_blocks[a] >= 1, "num_blocks must be >= 1");
-            require(d->cells_in_block[a] >= 1, "cells_in_block must be >= 1");
-            require(d->hi[a] > d->lo[a], "bounds must satisfy hi > lo");
-            require(d->map[a].kind == SPD_MAP_IDENTITY || d->map[a].kind == SPD_MAP_TANH, "unknown coordinate map");
-            if (d->map[a].kind == SPD_MAP_TANH)
-                require(d->map[a].p1 > d->map[a].p0 && d->map[a].beta > 0.0, "tanh map needs p1 > p0 and beta > 0");
-        }
-        require(d->num_exch >= 1 && d->num_exch <= 4, "num_exch must be in 1..4");
-        for (int a = 0; a < 3; ++a) require(d->cells_in_block[a] >= d->num_exch, "blocks must be at least num_exch cells wide");
-        // per-block kernels put the block index in the second grid dimension
-        require((long long)d->num_blocks[0]*d->num_blocks[1]*d->num_blocks[2] <= 65535, "at most 65535 blocks per grid");
+        require(ctx && out, "null argument");
+        validate_grid_desc(d);
         auto* G = new spd_grid_s;
         std::unique_ptr<spd_grid_s> guard(G);
         Grid& g = G->g;
@@ -229,8 +195,6 @@ int spd_grid_create(spd_ctx_t ctx, const spd_grid_desc* d, spd_grid_t* out)
         dg.sv = (long long)dg.N[0]*dg.N[1]*dg.N[2];
         dg.sb = ((5*dg.sv + 31)/32)*32;   // 256-byte aligned blocks
         // geometry tables (grid/cartesian_blocks.h:38-71, cartesian_grid.h:116-145)
-        double bsize[3];
-        for (int a = 0; a < 3; ++a) bsize[a] = (d->hi[a] - d->lo[a])/d->num_blocks[a];
         g.h_dx.resize(3*dg.nblk); g.h_inv_dx.resize(3*dg.nblk); g.h_bmin.resize(3*dg.nblk);
         std::vector<int> nbr(27*dg.nblk, -1);
         for (int lb = 0; lb < dg.nblk; ++lb)
@@ -238,10 +202,7 @@ int spd_grid_create(spd_ctx_t ctx, const spd_grid_desc* d, spd_grid_t* out)
             const int b[3] = {lb % dg.nb[0], (lb / dg.nb[0]) % dg.nb[1], lb / (dg.nb[0]*dg.nb[1])};
             for (int a = 0; a < 3; ++a)
             {
-                const double mn = d->lo[a] + b[a]*bsize[a];
-                const double mx = mn + bsize[a];
-                g.h_bmin[3*lb + a] = mn;
-                g.h_dx[3*lb + a] = (mx - mn)/dg.nc[a];
+                block_box(*d, lb, a, g.h_bmin[3*lb + a], g.h_dx[3*lb + a]);
                 g.h_inv_dx[3*lb + a] = 1.0/g.h_dx[3*lb + a];
             }
             // ghost region e of block lb is fed by block lb+e (wrapped); skipped
@@ -324,17 +285,7 @@ int spd_array_upload(spd_array_t arr, const double* host)
     return guarded([&]
     {
         require(arr && host, "null argument");
-        Grid& g = *arr->a.grid;
-        double* stage = ensure_stage(g);
-        const long long cells = (long long)g.geom.N[0]*g.geom.N[1]*g.geom.N[2];
-        for (int lb0 = 0; lb0 < g.geom.nblk; lb0 += g.stage_blocks)
-        {
-            const int nb = std::min(g.stage_blocks, g.geom.nblk - lb0);
-            SPD_HIP(hipMemcpyAsync(stage, host + 5*cells*lb0, sizeof(double)*5*cells*nb, hipMemcpyHostToDevice,
-                                   g.ctx->stream));
-            launch_aos_to_soa(g, stage, arr->a.d, lb0, nb, g.ctx->stream);
-        }
-        SPD_HIP(hipStreamSynchronize(g.ctx->stream));
+        upload_blocks(arr, 0, arr->a.grid->geom.nblk, host);
     });
 }
 
@@ -343,17 +294,7 @@ int spd_array_download(spd_array_t arr, double* host)
     return guarded([&]
     {
         require(arr && host, "null argument");
-        Grid& g = *arr->a.grid;
-        double* stage = ensure_stage(g);
-        const long long cells = (long long)g.geom.N[0]*g.geom.N[1]*g.geom.N[2];
-        for (int lb0 = 0; lb0 < g.geom.nblk; lb0 += g.stage_blocks)
-        {
-            const int nb = std::min(g.stage_blocks, g.geom.nblk - lb0);
-            launch_soa_to_aos(g, arr->a.d, stage, lb0, nb, g.ctx->stream);
-            SPD_HIP(hipMemcpyAsync(host + 5*cells*lb0, stage, sizeof(double)*5*cells*nb, hipMemcpyDeviceToHost,
-                                   g.ctx->stream));
-        }
-        SPD_HIP(hipStreamSynchronize(g.ctx->stream));
+        download_blocks(arr, 0, arr->a.grid->geom.nblk, host);
     });
 }
 
@@ -648,3 +589,64 @@ int spd_batch_set_shape(spd_batch_t b, int grid, int interleaved)
 }
 
 } // extern "C"
+
+namespace spd {
+
+void validate_grid_desc(const spd_grid_desc* d)
+{
+    require(d != nullptr, "grid description is null");
+    for (int a = 0; a < 3; ++a)
+    {
+        require(d->num_blocks[a] >= 1, "num_blocks must be >= 1");
+        require(d->cells_in_block[a] >= 1, "cells_in_block must be >= 1");
+        require(d->hi[a] > d->lo[a], "bounds must satisfy hi > lo");
+        require(d->map[a].kind == SPD_MAP_IDENTITY || d->map[a].kind == SPD_MAP_TANH, "unknown coordinate map");
+        if (d->map[a].kind == SPD_MAP_TANH)
+            require(d->map[a].p1 > d->map[a].p0 && d->map[a].beta > 0.0, "tanh map needs p1 > p0 and beta > 0");
+    }
+    require(d->num_exch >= 1 && d->num_exch <= 4, "num_exch must be in 1..4");
+    for (int a = 0; a < 3; ++a) require(d->cells_in_block[a] >= d->num_exch, "blocks must be at least num_exch cells wide");
+    // per-block kernels put the block index in the second grid dimension
+    require((long long)d->num_blocks[0]*d->num_blocks[1]*d->num_blocks[2] <= 65535, "at most 65535 blocks per grid");
+}
+
+double map_coord(const spd_map1d& m, double xi)
+{
+    if (m.kind != SPD_MAP_TANH) return xi;
+    const double c = 0.5*(m.p1 + m.p0), h = 0.5*(m.p1 - m.p0);
+    return c + h*std::tanh(m.beta*((xi - c)/h))/std::tanh(m.beta);
+}
+
+void upload_blocks(spd_array_s* arr, int lb0, int nb, const double* host)
+{
+    Grid& g = *arr->a.grid;
+    require(lb0 >= 0 && nb >= 0 && lb0 + nb <= g.geom.nblk, "block range outside the grid");
+    double* stage = ensure_stage(g);
+    const long long cells = (long long)g.geom.N[0]*g.geom.N[1]*g.geom.N[2];
+    for (int b0 = lb0; b0 < lb0 + nb; b0 += g.stage_blocks)
+    {
+        const int n = std::min(g.stage_blocks, lb0 + nb - b0);
+        SPD_HIP(hipMemcpyAsync(stage, host + 5*cells*(b0 - lb0), sizeof(double)*5*cells*n, hipMemcpyHostToDevice,
+                               g.ctx->stream));
+        launch_aos_to_soa(g, stage, arr->a.d, b0, n, g.ctx->stream);
+    }
+    SPD_HIP(hipStreamSynchronize(g.ctx->stream));
+}
+
+void download_blocks(const spd_array_s* arr, int lb0, int nb, double* host)
+{
+    Grid& g = *arr->a.grid;
+    require(lb0 >= 0 && nb >= 0 && lb0 + nb <= g.geom.nblk, "block range outside the grid");
+    double* stage = ensure_stage(g);
+    const long long cells = (long long)g.geom.N[0]*g.geom.N[1]*g.geom.N[2];
+    for (int b0 = lb0; b0 < lb0 + nb; b0 += g.stage_blocks)
+    {
+        const int n = std::min(g.stage_blocks, lb0 + nb - b0);
+        launch_soa_to_aos(g, arr->a.d, stage, b0, n, g.ctx->stream);
+        SPD_HIP(hipMemcpyAsync(host + 5*cells*(b0 - lb0), stage, sizeof(double)*5*cells*n, hipMemcpyDeviceToHost,
+                               g.ctx->stream));
+    }
+    SPD_HIP(hipStreamSynchronize(g.ctx->stream));
+}
+
+} // namespace spd

@@ -1,0 +1,66 @@
+// Handles and error plumbing shared by the translation units of the C ABI.
+#pragma once
+#include "spd_internal.hpp"
+
+struct spd_ctx_s   { spd::Ctx c; };
+struct spd_grid_s  { spd::Grid g; };
+struct spd_array_s { spd::Array a; };
+
+namespace spd {
+
+// a file could not be opened, read or written
+struct IoError { std::string msg; };
+
+void set_error(const std::string& msg);
+
+inline void require(bool ok, const std::string& msg) { if (!ok) throw ArgError{msg}; }
+
+// Runs fn and maps what it throws to the SPD_* status codes (the message is
+// kept for spd_last_error()).
+template <typename F> int guarded(F&& fn)
+{
+    try
+    {
+        fn();
+        return SPD_OK;
+    }
+    catch (const ArgError& e) { set_error(e.msg); return SPD_ERR_ARG; }
+    catch (const IoError& e) { set_error(e.msg); return SPD_ERR_IO; }
+    catch (const HipError& e)
+    {
+        set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);
+        return e.code == hipErrorOutOfMemory ? SPD_ERR_MEMORY : SPD_ERR_HIP;
+    }
+    catch (const std::bad_alloc&) { set_error("host allocation failed"); return SPD_ERR_MEMORY; }
+    catch (const std::exception& e) { set_error(e.what()); return SPD_ERR_ARG; }
+}
+
+// Checks a grid description (sizes, bounds, maps, ghost layers).
+void validate_grid_desc(const spd_grid_desc* d);
+
+// Box minimum and cell size of block lb along axis a, as cartesian_blocks_t
+// and grid_geometry_t compute them (grid/cartesian_blocks.h:46-71,
+// grid/grid_geometry.h:74-85): bsize = (hi - lo)/nb, min = lo + b bsize,
+// max = min + bsize, dx = (max - min)/nc.
+inline void block_box(const spd_grid_desc& d, int lb, int a, double& mn, double& dx)
+{
+    const int b[3] = {lb % d.num_blocks[0], (lb / d.num_blocks[0]) % d.num_blocks[1],
+                      lb / (d.num_blocks[0]*d.num_blocks[1])};
+    const double bsize = (d.hi[a] - d.lo[a])/d.num_blocks[a];
+    mn = d.lo[a] + b[a]*bsize;
+    const double mx = mn + bsize;
+    dx = (mx - mn)/d.cells_in_block[a];
+}
+
+// Physical coordinate of computational coordinate xi along axis a: identity,
+// or x = c + h tanh(beta (xi - c)/h)/tanh(beta) with c, h the centre and
+// half-width of [p0, p1].
+double map_coord(const spd_map1d& m, double xi);
+
+// Blocks [lb0, lb0 + nb) of an array from / to host memory in the reference
+// layout (AoS, variable fastest, ghosts stored), through the grid's staging
+// buffer; both return when the host memory may be reused.
+void upload_blocks(spd_array_s* arr, int lb0, int nb, const double* host);
+void download_blocks(const spd_array_s* arr, int lb0, int nb, double* host);
+
+} // namespace spd

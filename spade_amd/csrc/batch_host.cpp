@@ -295,21 +295,35 @@ const BatchCase& Batch::get(int id) const
     return it->second;
 }
 
+// Calls f(b, e) for each run slots[b..e) of consecutive slot indices, so that
+// blocks move in one copy per run (a case admitted into a fresh pool is one run).
+template <class F>
+static void for_slot_runs(const std::vector<int>& slots, F f)
+{
+    for (size_t b = 0, e; b < slots.size(); b = e)
+    {
+        for (e = b + 1; e < slots.size() && slots[e] == slots[e - 1] + 1; ++e) {}
+        f(b, e);
+    }
+}
+
 void Batch::download(int id, double* host) const
 {
     const BatchCase& c = get(id);
-    for (size_t b = 0; b < c.slots.size(); ++b)
+    for_slot_runs(c.slots, [&](size_t b, size_t e) {
         SPD_HIP(hipMemcpyAsync(host + b*kSlotDoubles, Q[0] + (size_t)c.slots[b]*kSlotDoubles,
-                               sizeof(double)*kSlotDoubles, hipMemcpyDeviceToHost, ctx->stream));
+                               sizeof(double)*kSlotDoubles*(e - b), hipMemcpyDeviceToHost, ctx->stream));
+    });
     SPD_HIP(hipStreamSynchronize(ctx->stream));
 }
 
 void Batch::upload(int id, const double* host)
 {
     const BatchCase& c = get(id);
-    for (size_t b = 0; b < c.slots.size(); ++b)
+    for_slot_runs(c.slots, [&](size_t b, size_t e) {
         SPD_HIP(hipMemcpyAsync(Q[0] + (size_t)c.slots[b]*kSlotDoubles, host + b*kSlotDoubles,
-                               sizeof(double)*kSlotDoubles, hipMemcpyHostToDevice, ctx->stream));
+                               sizeof(double)*kSlotDoubles*(e - b), hipMemcpyHostToDevice, ctx->stream));
+    });
     SPD_HIP(hipStreamSynchronize(ctx->stream));
 }
 
@@ -324,10 +338,11 @@ int Batch::tune()
     // save the step-start state (Q0) of the active slots
     double* save = nullptr;
     SPD_HIP(hipMalloc((void**)&save, sizeof(double)*kSlotDoubles*(size_t)nwork));
-    for (int w = 0; w < nwork; ++w)
-        SPD_HIP(hipMemcpyAsync(save + (size_t)w*kSlotDoubles, Q[0] + (size_t)h_worklist[w]*kSlotDoubles,
-                               sizeof(double)*kSlotDoubles, hipMemcpyDeviceToDevice, ctx->stream));
     const std::vector<int> saved_slots = h_worklist;
+    for_slot_runs(saved_slots, [&](size_t b, size_t e) {
+        SPD_HIP(hipMemcpyAsync(save + b*kSlotDoubles, Q[0] + (size_t)saved_slots[b]*kSlotDoubles,
+                               sizeof(double)*kSlotDoubles*(e - b), hipMemcpyDeviceToDevice, ctx->stream));
+    });
     int saved_step = 0;
     SPD_HIP(hipMemcpyAsync(&saved_step, d_small + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     SPD_HIP(hipStreamSynchronize(ctx->stream));
@@ -354,9 +369,10 @@ int Batch::tune()
     grid = best_grid;
     order_interleaved = best_order;
     upload_worklist();
-    for (size_t w = 0; w < saved_slots.size(); ++w)
-        SPD_HIP(hipMemcpyAsync(Q[0] + (size_t)saved_slots[w]*kSlotDoubles, save + w*kSlotDoubles,
-                               sizeof(double)*kSlotDoubles, hipMemcpyDeviceToDevice, ctx->stream));
+    for_slot_runs(saved_slots, [&](size_t b, size_t e) {
+        SPD_HIP(hipMemcpyAsync(Q[0] + (size_t)saved_slots[b]*kSlotDoubles, save + b*kSlotDoubles,
+                               sizeof(double)*kSlotDoubles*(e - b), hipMemcpyDeviceToDevice, ctx->stream));
+    });
     SPD_HIP(hipMemcpyAsync(d_small + 1, &saved_step, sizeof(int), hipMemcpyHostToDevice, ctx->stream));
     SPD_HIP(hipStreamSynchronize(ctx->stream));
     SPD_HIP(hipFree(save));

@@ -61,6 +61,13 @@ int main(int argc, char** argv)
         sa::integrator_t it(q, flux, dt, sa::ssprk3);
         it.advance(3);
         write(out + "/facade_step3.bin", q.download());
+        // io::binary_write / binary_read round trip and io::output_vtk
+        sa::io::binary_write(out + "/facade_step3_checkpoint.bin", q);
+        sa::grid_array back(g);
+        back = 0.0;
+        sa::io::binary_read(out + "/facade_step3_checkpoint.bin", back);
+        write(out + "/facade_step3_readback.bin", back.download());
+        sa::io::output_vtk(out + "/vtk", "step3", q);
         std::printf("dt %.17g\n", dt);
     }
     catch (const std::exception& e)

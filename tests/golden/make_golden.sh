@@ -8,4 +8,4 @@ here="$(cd "$(dirname "$0")" && pwd)"
 root="$(cd "$here/../.." && pwd)"
 make -C "$root/oracle/ref" REF=/root/reference/src
 "$root/oracle/_ref/ref_driver" "$here"
-( cd "$here" && sha256sum *.bin > SHA256SUMS )
+( cd "$here" && sha256sum *.bin vtk_g1/*.pvts vtk_g1/data_g1/*.vts > SHA256SUMS )

@@ -41,3 +41,11 @@ def test_facade_demo_matches_oracle():
     err = gi.rel_maxnorm(gi.interior(step3, 2), gi.interior(q, 2))
     print("facade 3 steps rel err", err)
     assert np.all(err <= 1e-12)
+    # io::binary_write wrote the host image; binary_read restored it; output_vtk wrote every block
+    with open(os.path.join(d, "facade_step3.bin"), "rb") as a, \
+            open(os.path.join(d, "facade_step3_checkpoint.bin"), "rb") as b, \
+            open(os.path.join(d, "facade_step3_readback.bin"), "rb") as c:
+        img = a.read()
+        assert b.read() == img and c.read() == img
+    assert os.path.exists(os.path.join(d, "vtk", "step3.pvts"))
+    assert sorted(os.listdir(os.path.join(d, "vtk", "data_step3"))) == [f"b{lb:09d}.vts" for lb in range(8)]
