@@ -61,6 +61,7 @@ public:
     const grid_desc& desc() const { return d_; }
     // doubles in the reference-layout host image of one array
     long host_size() const { return spd_array_host_size(h_); }
+    long device_size() const { return spd_array_device_size(h_); }
     int num_blocks() const { return d_.num_blocks[0]*d_.num_blocks[1]*d_.num_blocks[2]; }
 };
 

@@ -146,6 +146,11 @@ class CartesianGrid:
         return c[0] * c[1] * c[2] * self.nblocks
 
     @property
+    def device_size(self):
+        """Doubles one array of this grid occupies on the device (padded rows)."""
+        return int(load().spd_array_device_size(self._h))
+
+    @property
     def host_shape(self):
         """Shape of the reference-layout host image: (blocks, Nz, Ny, Nx, 5)."""
         ng = self.num_exch

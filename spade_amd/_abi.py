@@ -76,6 +76,7 @@ SIGNATURES = {
     "spd_array_fill": (C.c_int, [_vp, C.c_double]),
     "spd_array_copy": (C.c_int, [_vp, _vp]),
     "spd_array_host_size": (C.c_long, [_vp]),
+    "spd_array_device_size": (C.c_long, [_vp]),
     "spd_flux_div": (C.c_int, [_vp, _vp, C.POINTER(FluxDesc), C.c_int]),
     "spd_flux_div_kernel": (C.c_int, [_vp, C.POINTER(FluxDesc), C.POINTER(C.c_int)]),
     "spd_exchange": (C.c_int, [_vp]),

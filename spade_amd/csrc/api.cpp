@@ -192,7 +192,8 @@ int spd_grid_create(spd_ctx_t ctx, const spd_grid_desc* d, spd_grid_t* out)
             dg.map_tanhb[a] = d->map[a].kind == SPD_MAP_TANH ? std::tanh(d->map[a].beta) : 1.0;
         }
         dg.identity = (dg.map_kind[0] == SPD_MAP_IDENTITY && dg.map_kind[1] == SPD_MAP_IDENTITY && dg.map_kind[2] == SPD_MAP_IDENTITY);
-        dg.sv = (long long)dg.N[0]*dg.N[1]*dg.N[2];
+        dg.P = row_pitch(dg.N[0]);
+        dg.sv = (long long)dg.P*dg.N[1]*dg.N[2];
         dg.sb = ((5*dg.sv + 31)/32)*32;   // 256-byte aligned blocks
         // geometry tables (grid/cartesian_blocks.h:38-71, cartesian_grid.h:116-145)
         g.h_dx.resize(3*dg.nblk); g.h_inv_dx.resize(3*dg.nblk); g.h_bmin.resize(3*dg.nblk);
@@ -261,6 +262,13 @@ long spd_array_host_size(spd_grid_t grid)
     if (!grid) return 0;
     const DevGeom& G = grid->g.geom;
     return 5L*G.N[0]*G.N[1]*G.N[2]*G.nblk;
+}
+
+long spd_array_device_size(spd_grid_t grid)
+{
+    if (!grid) return 0;
+    const DevGeom& G = grid->g.geom;
+    return (long)G.sb*G.nblk;
 }
 
 int spd_array_create(spd_grid_t grid, spd_array_t* out)

@@ -319,8 +319,8 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     const double ix = G.inv_dx[3*lb + 0], iy = G.inv_dx[3*lb + 1], iz = G.inv_dx[3*lb + 2];
     const int nz = G.nc[2];
     const long long sv = G.sv;
-    const int N0 = G.N[0], N1 = G.N[1], ng = G.ng;
-    const long long plane = (long long)N0*N1;
+    const int P = G.P, N1 = G.N[1], ng = G.ng;
+    const long long plane = (long long)P*N1;
 
     // halo cell of this thread: waves 0 and 1 own the 64 edge cells and the
     // 4 corners of the 18x18 plane (waves 2 and 3 evaluate the block-boundary faces)
@@ -335,9 +335,9 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
 
     // column bases at plane k = 0 (stored plane ng); plane k is base + k*plane
     const long long blk = (long long)lb*G.sb;
-    const double* __restrict__ Bown = prims + blk + ((long long)ng*N1 + (j + ng))*N0 + (i + ng);
-    const double* __restrict__ Bhalo = prims + blk + ((long long)ng*N1 + (hj + ng))*N0 + (hi + ng);
-    double* __restrict__ Rown = rhs + blk + ((long long)ng*N1 + (j + ng))*N0 + (i + ng);
+    const double* __restrict__ Bown = prims + blk + ((long long)ng*N1 + (j + ng))*P + (i + ng);
+    const double* __restrict__ Bhalo = prims + blk + ((long long)ng*N1 + (hj + ng))*P + (hi + ng);
+    double* __restrict__ Rown = rhs + blk + ((long long)ng*N1 + (j + ng))*P + (i + ng);
 
     auto load_at = [&](const double* __restrict__ col, int ck) -> Raw
     {

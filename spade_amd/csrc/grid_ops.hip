@@ -64,7 +64,14 @@ k_wall_fill(DevGeom G, double* __restrict__ a, int idir, int pm, double t_wall)
 
 // Layout conversion on the device for nb blocks starting at block lb0:
 // stage holds them in the reference's host layout (AoS, variable fastest,
-// stored cells including ghosts, mem_map.h:484-496), a is the device array.
+// stored cells including ghosts, mem_map.h:484-496), a is the device array
+// (stored cell c = (i, j, k) at row (j, k) of pitch P).
+__device__ inline long long stored_offset(const DevGeom& G, int c)
+{
+    const int i = c % G.N[0], r = c / G.N[0];     // r = j + N1 k
+    return (long long)r*G.P + i;
+}
+
 __global__ void __launch_bounds__(256)
 k_aos_to_soa(DevGeom G, const double* __restrict__ stage, double* __restrict__ a, int lb0)
 {
@@ -73,7 +80,7 @@ k_aos_to_soa(DevGeom G, const double* __restrict__ stage, double* __restrict__ a
     if (c >= cells) return;
     const int b = blockIdx.y;
     const double* e = stage + 5*((long long)b*cells + c);
-    double* d = a + (long long)(lb0 + b)*G.sb + c;
+    double* d = a + (long long)(lb0 + b)*G.sb + stored_offset(G, c);
     #pragma unroll
     for (int v = 0; v < 5; ++v) d[v*G.sv] = e[v];
 }
@@ -86,7 +93,7 @@ k_soa_to_aos(DevGeom G, const double* __restrict__ a, double* __restrict__ stage
     if (c >= cells) return;
     const int b = blockIdx.y;
     double* e = stage + 5*((long long)b*cells + c);
-    const double* d = a + (long long)(lb0 + b)*G.sb + c;
+    const double* d = a + (long long)(lb0 + b)*G.sb + stored_offset(G, c);
     #pragma unroll
     for (int v = 0; v < 5; ++v) e[v] = d[v*G.sv];
 }

@@ -32,9 +32,11 @@ struct DevGeom
     int nc[3];        // cells per block
     int ng;           // ghost layers
     int N[3];         // stored extent nc + 2 ng
+    int P;            // row pitch in doubles: the N0 stored cells of a row, then
+                      // padding to a multiple of 16 doubles (128 B)
     int nb[3];        // blocks per axis
     int nblk;
-    long long sv;     // doubles between variables of one block (N0 N1 N2)
+    long long sv;     // doubles between variables of one block (P N1 N2)
     long long sb;     // doubles between blocks (5 sv, padded)
     int identity;     // all three maps identity
     int map_kind[3];
@@ -72,14 +74,15 @@ struct Array
 };
 
 // Offset of an array's data from its 256-byte aligned allocation, chosen so
-// that the interior part of every row (i = 0 .. nc0-1) starts on a 32-byte
-// boundary: with a row stride N0 that is a multiple of 4 doubles, a lead of
-// -ng mod 4 doubles aligns all of them, and whole-row stores of the interior
-// then cover full 32-byte sectors instead of partial ones at both ends.
+// that the interior part of every row (i = 0 .. nc0-1) starts on a 128-byte
+// boundary: with a row pitch P that is a multiple of 16 doubles, a lead of
+// -ng mod 16 doubles aligns all of them.  Row stores that are not aligned to
+// 64 bytes run at a third of the aligned rate on MI355X HBM (partial 64-byte
+// writes), and a 16-cell row on a 128-byte line is one request instead of two.
+inline int row_pitch(int N0) { return ((N0 + 15)/16)*16; }
 inline int lead_doubles(const DevGeom& g)
 {
-    if (g.N[0] % 4 != 0) return 0;
-    return (4 - g.ng % 4) % 4;
+    return (16 - g.ng % 16) % 16;
 }
 
 // ---- device helpers shared by the kernels ----
@@ -127,7 +130,7 @@ double reduce_max_wavespeed(const Grid& g, const double* q, double gamma, double
 __host__ __device__ inline long long cell_index(const DevGeom& g, int lb, int v, int i, int j, int k)
 {
     return (long long)lb*g.sb + (long long)v*g.sv
-         + ((long long)(k + g.ng)*g.N[1] + (j + g.ng))*g.N[0] + (i + g.ng);
+         + ((long long)(k + g.ng)*g.N[1] + (j + g.ng))*g.P + (i + g.ng);
 }
 
 } // namespace spd
