@@ -83,10 +83,9 @@ spd_array_s* new_array(Grid& g)
     auto* A = new spd_array_s;
     std::unique_ptr<spd_array_s> guard(A);
     A->a.grid = &g;
-    const int lead = lead_doubles(g.geom);
-    const size_t bytes = sizeof(double)*((size_t)g.geom.sb*g.geom.nblk + lead);
+    const size_t bytes = sizeof(double)*(size_t)g.geom.sb*g.geom.nblk;
     SPD_HIP(hipMalloc((void**)&A->a.base, bytes));
-    A->a.d = A->a.base + lead;
+    A->a.d = A->a.base;
     SPD_HIP(hipMemsetAsync(A->a.base, 0, bytes, g.ctx->stream));
     return guard.release();
 }
@@ -192,8 +191,8 @@ int spd_grid_create(spd_ctx_t ctx, const spd_grid_desc* d, spd_grid_t* out)
             dg.map_tanhb[a] = d->map[a].kind == SPD_MAP_TANH ? std::tanh(d->map[a].beta) : 1.0;
         }
         dg.identity = (dg.map_kind[0] == SPD_MAP_IDENTITY && dg.map_kind[1] == SPD_MAP_IDENTITY && dg.map_kind[2] == SPD_MAP_IDENTITY);
-        dg.P = row_pitch(dg.N[0]);
-        dg.sv = (long long)dg.P*dg.N[1]*dg.N[2];
+        dg.sv = (long long)dg.N[0]*dg.N[1]*dg.N[2];
+        dg.xh = (long long)dg.N[1]*dg.N[2]*dg.nc[0];
         dg.sb = ((5*dg.sv + 31)/32)*32;   // 256-byte aligned blocks
         // geometry tables (grid/cartesian_blocks.h:38-71, cartesian_grid.h:116-145)
         g.h_dx.resize(3*dg.nblk); g.h_inv_dx.resize(3*dg.nblk); g.h_bmin.resize(3*dg.nblk);

@@ -319,8 +319,9 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     const double ix = G.inv_dx[3*lb + 0], iy = G.inv_dx[3*lb + 1], iz = G.inv_dx[3*lb + 2];
     const int nz = G.nc[2];
     const long long sv = G.sv;
-    const int P = G.P, N1 = G.N[1], ng = G.ng;
-    const long long plane = (long long)P*N1;
+    const int N1 = G.N[1], ng = G.ng;
+    // plane (k) strides of a core column and of an x-halo column
+    const long long plane = (long long)G.nc[0]*N1, xplane = (long long)2*ng*N1;
 
     // halo cell of this thread: waves 0 and 1 own the 64 edge cells and the
     // 4 corners of the 18x18 plane (waves 2 and 3 evaluate the block-boundary faces)
@@ -333,15 +334,18 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     else if (tid < 68) { const int c = tid - 64; hi = (c & 1) ? 16 : -1; hj = (c & 2) ? 16 : -1; }
     const int pown = lpos(i, j), phalo = lpos(hi, hj);
 
-    // column bases at plane k = 0 (stored plane ng); plane k is base + k*plane
+    // column bases at plane k = 0; plane k is base + k*stride (the own column
+    // and the residual are core columns, a halo column with hi = -1 or 16 is
+    // an x-halo column)
     const long long blk = (long long)lb*G.sb;
-    const double* __restrict__ Bown = prims + blk + ((long long)ng*N1 + (j + ng))*P + (i + ng);
-    const double* __restrict__ Bhalo = prims + blk + ((long long)ng*N1 + (hj + ng))*P + (hi + ng);
-    double* __restrict__ Rown = rhs + blk + ((long long)ng*N1 + (j + ng))*P + (i + ng);
+    const double* __restrict__ Bown = prims + blk + stored_offset(G, i, j, 0);
+    const double* __restrict__ Bhalo = prims + blk + stored_offset(G, hi, hj, 0);
+    const long long hplane = (hi >= 0 && hi < 16) ? plane : xplane;
+    double* __restrict__ Rown = rhs + blk + stored_offset(G, i, j, 0);
 
-    auto load_at = [&](const double* __restrict__ col, int ck) -> Raw
+    auto load_at = [&](const double* __restrict__ col, int ck, long long stride) -> Raw
     {
-        const double* e = col + ck*plane;
+        const double* e = col + ck*stride;
         return Raw{e[0], e[sv], e[2*sv], e[3*sv], e[4*sv]};
     };
     auto cell = [&](const Raw& r) -> CellV { return make_cell(C, r.p, r.T, r.u, r.v, r.w); };
@@ -363,24 +367,24 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     // prologue: planes -1, 0, 1 resident; planes 2 and 3 in flight (two raw
     // register sets used alternately, so a plane's load is never copied and
     // waiting for it never waits for the later one)
-    const CellV prev = cell(load_at(Bown, -1));
-    const CellV cur0 = cell(load_at(Bown, 0));
-    const CellV nxt0 = cell(load_at(Bown, 1));
+    const CellV prev = cell(load_at(Bown, -1, plane));
+    const CellV cur0 = cell(load_at(Bown, 0, plane));
+    const CellV nxt0 = cell(load_at(Bown, 1, plane));
     Raw nA{}, nB{}, hA{}, hB{};
-    if (2 <= nz) nA = load_at(Bown, 2);
-    if (3 <= nz) nB = load_at(Bown, 3);
+    if (2 <= nz) nA = load_at(Bown, 2, plane);
+    if (3 <= nz) nB = load_at(Bown, 3, plane);
     put_vel(ring[0], pown, prev);
     put_vel(ring[1], pown, cur0); put_sca(sS[0], pown, cur0);
     put_vel(ring[2], pown, nxt0); put_sca(sS[1], pown, nxt0);
     if (tid == 0) { sBad[0] = 0; sBad[1] = 0; }
     if (has_halo)
     {
-        put_vel(ring[0], phalo, cell(load_at(Bhalo, -1)));
-        const CellV h0 = cell(load_at(Bhalo, 0)), h1 = cell(load_at(Bhalo, 1));
+        put_vel(ring[0], phalo, cell(load_at(Bhalo, -1, hplane)));
+        const CellV h0 = cell(load_at(Bhalo, 0, hplane)), h1 = cell(load_at(Bhalo, 1, hplane));
         put_vel(ring[1], phalo, h0); put_sca(sS[0], phalo, h0);
         put_vel(ring[2], phalo, h1); put_sca(sS[1], phalo, h1);
-        if (2 <= nz) hA = load_at(Bhalo, 2);
-        if (3 <= nz) hB = load_at(Bhalo, 3);
+        if (2 <= nz) hA = load_at(Bhalo, 2, hplane);
+        if (3 <= nz) hB = load_at(Bhalo, 3, hplane);
     }
     lds_barrier();
 
@@ -499,12 +503,12 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             }
             const CellV c2 = cell(nr);
             put_vel(s_new, pown, c2); put_sca(t_new, pown, c2);
-            if (k + 4 <= nz) nr = load_at(Bown, k + 4);
+            if (k + 4 <= nz) nr = load_at(Bown, k + 4, plane);
             if (has_halo)
             {
                 const CellV h = cell(hr);
                 put_vel(s_new, phalo, h); put_sca(t_new, phalo, h);
-                if (k + 4 <= nz) hr = load_at(Bhalo, k + 4);
+                if (k + 4 <= nz) hr = load_at(Bhalo, k + 4, hplane);
             }
         }
         lds_barrier();

@@ -27,7 +27,7 @@ struct Cell
     int lb;
     __device__ inline double val(int v, int i, int j, int k) const
     {
-        return B[(long long)v*G->sv + ((long long)(k + G->ng)*G->N[1] + (j + G->ng))*G->P + (i + G->ng)];
+        return B[(long long)v*G->sv + stored_offset(*G, i, j, k)];
     }
     __device__ inline Prim load(int i, int j, int k) const
     {

@@ -64,12 +64,11 @@ k_wall_fill(DevGeom G, double* __restrict__ a, int idir, int pm, double t_wall)
 
 // Layout conversion on the device for nb blocks starting at block lb0:
 // stage holds them in the reference's host layout (AoS, variable fastest,
-// stored cells including ghosts, mem_map.h:484-496), a is the device array
-// (stored cell c = (i, j, k) at row (j, k) of pitch P).
+// stored cells including ghosts, mem_map.h:484-496), a is the device array.
 __device__ inline long long stored_offset(const DevGeom& G, int c)
 {
-    const int i = c % G.N[0], r = c / G.N[0];     // r = j + N1 k
-    return (long long)r*G.P + i;
+    const int i = c % G.N[0], r = c / G.N[0];     // r = (j + ng) + N1 (k + ng)
+    return stored_offset(G, i - G.ng, r % G.N[1] - G.ng, r / G.N[1] - G.ng);
 }
 
 __global__ void __launch_bounds__(256)

@@ -58,7 +58,7 @@ k_rk_transform(DevGeom G, double* __restrict__ q, Coefs C, double gamma, double 
     if (!interior_cell(G, ci)) return;
     double qq[5], cons[5];
     long long idx[5];
-    for (int v = 0; v < 5; ++v) { idx[v] = cell_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
+    for (int v = 0; v < 5; ++v) { idx[v] = core_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
     p2c(gamma, R, qq, cons);
     #pragma unroll
     for (int s = 0; s < 3; ++s)               // static indices: C stays in scalar registers
@@ -78,7 +78,7 @@ k_rk_opt_s0(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, do
     CellIt ci;
     if (!interior_cell(G, ci)) return;
     double qq[5], c[5]; long long idx[5];
-    for (int v = 0; v < 5; ++v) { idx[v] = cell_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
+    for (int v = 0; v < 5; ++v) { idx[v] = core_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
     p2c(gamma, R, qq, c);
     for (int v = 0; v < 5; ++v) c[v] += dt*r0[idx[v]];
     c2p(gamma, R, c, qq);
@@ -91,7 +91,7 @@ k_rk_opt_s1(DevGeom G, double* __restrict__ q, double* __restrict__ r0, const do
     CellIt ci;
     if (!interior_cell(G, ci)) return;
     double qq[5], c[5], old[5], nr[5]; long long idx[5];
-    for (int v = 0; v < 5; ++v) { idx[v] = cell_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
+    for (int v = 0; v < 5; ++v) { idx[v] = core_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
     p2c(gamma, R, qq, c);
     const double sixth_dt = (1.0/6.0)*dt;
     for (int v = 0; v < 5; ++v) { old[v] = r0[idx[v]]; nr[v] = sixth_dt*(old[v] + r1[idx[v]]); r0[idx[v]] = nr[v]; }
@@ -107,7 +107,7 @@ k_rk_opt_s2(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, co
     CellIt ci;
     if (!interior_cell(G, ci)) return;
     double qq[5], c[5]; long long idx[5];
-    for (int v = 0; v < 5; ++v) { idx[v] = cell_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
+    for (int v = 0; v < 5; ++v) { idx[v] = core_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
     p2c(gamma, R, qq, c);
     const double two_thirds_dt = dt*(2.0/3.0);
     for (int v = 0; v < 5; ++v) c[v] -= (1.0/2.0)*r0[idx[v]];

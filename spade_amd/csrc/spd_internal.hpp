@@ -32,12 +32,11 @@ struct DevGeom
     int nc[3];        // cells per block
     int ng;           // ghost layers
     int N[3];         // stored extent nc + 2 ng
-    int P;            // row pitch in doubles: the N0 stored cells of a row, then
-                      // padding to a multiple of 16 doubles (128 B)
     int nb[3];        // blocks per axis
     int nblk;
-    long long sv;     // doubles between variables of one block (P N1 N2)
-    long long sb;     // doubles between blocks (5 sv, padded)
+    long long sv;     // doubles between variables of one block (N0 N1 N2)
+    long long sb;     // doubles between blocks (5 sv, padded to 256 bytes)
+    long long xh;     // start of the x-halo part of a variable (N1 N2 nc0)
     int identity;     // all three maps identity
     int map_kind[3];
     double map_c[3], map_h[3], map_beta[3], map_tanhb[3];
@@ -70,20 +69,9 @@ struct Array
 {
     Grid* grid = nullptr;
     double* base = nullptr;       // allocation
-    double* d = nullptr;          // device data, geom.sb * nblk doubles, at base + lead_doubles(geom)
+    double* d = nullptr;          // device data, geom.sb * nblk doubles (= base)
 };
 
-// Offset of an array's data from its 256-byte aligned allocation, chosen so
-// that the interior part of every row (i = 0 .. nc0-1) starts on a 128-byte
-// boundary: with a row pitch P that is a multiple of 16 doubles, a lead of
-// -ng mod 16 doubles aligns all of them.  Row stores that are not aligned to
-// 64 bytes run at a third of the aligned rate on MI355X HBM (partial 64-byte
-// writes), and a 16-cell row on a 128-byte line is one request instead of two.
-inline int row_pitch(int N0) { return ((N0 + 15)/16)*16; }
-inline int lead_doubles(const DevGeom& g)
-{
-    return (16 - g.ng % 16) % 16;
-}
 
 // ---- device helpers shared by the kernels ----
 
@@ -126,11 +114,33 @@ void launch_rk_opt_s2(const Grid& g, double* q, const double* r0, const double* 
 void diagnostics(const Grid& g, const double* q, double R, double& ke, double& en, hipStream_t s);
 double reduce_max_wavespeed(const Grid& g, const double* q, double gamma, double R, hipStream_t s);
 
+// Device layout of one variable of one block ("core + x-halo"):
+//   core:   every stored row (j, k in [-ng, n+ng)), ghost rows included, holds
+//           its nc0 interior-x cells (i = 0 .. nc0-1); rows in (j, k) order,
+//           pitch nc0 -- with nc0 = 16 a row is one 128-byte line;
+//   x-halo: the 2 ng x-ghost cells of every row (i = -ng .. -1, nc0 .. nc0+ng-1),
+//           rows in the same order, at offset xh.
+// Interior rows are whole, aligned lines (row stores that are not 64-byte
+// aligned run at a third of the aligned rate on MI355X HBM), the x-ghost
+// cells of consecutive rows are contiguous (the exchange writes whole lines),
+// and nothing is padding.
+__host__ __device__ inline long long stored_offset(const DevGeom& g, int i, int j, int k)
+{
+    const long long row = (long long)(k + g.ng)*g.N[1] + (j + g.ng);
+    if (i >= 0 && i < g.nc[0]) return row*g.nc[0] + i;
+    return g.xh + row*(2*g.ng) + (i < 0 ? i + g.ng : i - g.nc[0] + g.ng);
+}
+
 // Index of a stored cell (i, j, k in [-ng, n+ng)) of variable v in block lb.
 __host__ __device__ inline long long cell_index(const DevGeom& g, int lb, int v, int i, int j, int k)
 {
-    return (long long)lb*g.sb + (long long)v*g.sv
-         + ((long long)(k + g.ng)*g.N[1] + (j + g.ng))*g.P + (i + g.ng);
+    return (long long)lb*g.sb + (long long)v*g.sv + stored_offset(g, i, j, k);
+}
+
+// The same for a cell with 0 <= i < nc0 (core part only).
+__host__ __device__ inline long long core_index(const DevGeom& g, int lb, int v, int i, int j, int k)
+{
+    return (long long)lb*g.sb + (long long)v*g.sv + ((long long)(k + g.ng)*g.N[1] + (j + g.ng))*g.nc[0] + i;
 }
 
 } // namespace spd
