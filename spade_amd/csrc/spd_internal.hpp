@@ -118,17 +118,19 @@ double reduce_max_wavespeed(const Grid& g, const double* q, double gamma, double
 //   core:   every stored row (j, k in [-ng, n+ng)), ghost rows included, holds
 //           its nc0 interior-x cells (i = 0 .. nc0-1); rows in (j, k) order,
 //           pitch nc0 -- with nc0 = 16 a row is one 128-byte line;
-//   x-halo: the 2 ng x-ghost cells of every row (i = -ng .. -1, nc0 .. nc0+ng-1),
-//           rows in the same order, at offset xh.
+//   x-halo: at offset xh, one section per ghost layer (layer 0: i = -1 and
+//           nc0; layer 1: i = -2 and nc0+1; ...), each holding the (low, high)
+//           pair of every row in the same row order.
 // Interior rows are whole, aligned lines (row stores that are not 64-byte
 // aligned run at a third of the aligned rate on MI355X HBM), the x-ghost
-// cells of consecutive rows are contiguous (the exchange writes whole lines),
-// and nothing is padding.
+// cells of consecutive rows are contiguous (the exchange writes whole lines,
+// a one-layer stencil reads only the first section), and nothing is padding.
 __host__ __device__ inline long long stored_offset(const DevGeom& g, int i, int j, int k)
 {
     const long long row = (long long)(k + g.ng)*g.N[1] + (j + g.ng);
     if (i >= 0 && i < g.nc[0]) return row*g.nc[0] + i;
-    return g.xh + row*(2*g.ng) + (i < 0 ? i + g.ng : i - g.nc[0] + g.ng);
+    const int layer = (i < 0) ? -1 - i : i - g.nc[0];
+    return g.xh + (long long)layer*2*g.N[1]*g.N[2] + 2*row + (i < 0 ? 0 : 1);
 }
 
 // Index of a stored cell (i, j, k in [-ng, n+ng)) of variable v in block lb.
