@@ -70,10 +70,12 @@ double* ensure_stage(Grid& g)
     return g.d_stage;
 }
 
+// The boundary step after a stage (exchange + wall fill): the stage kernels
+// have already written the exchanged ghosts (push_ghosts), so only the walls
+// remain; they mirror cells that include exchanged x-ghosts, hence after.
 void apply_bc(spd_integrator_s* it)
 {
     Grid& g = *it->q->a.grid;
-    launch_exchange(g, it->q->a.d, g.ctx->stream);
     for (int face = 0; face < 6; ++face)
         if (it->bc.wall_mask & (1 << face)) launch_wall_fill(g, it->q->a.d, face, it->bc.t_wall, g.ctx->stream);
 }
@@ -227,12 +229,20 @@ int spd_grid_create(spd_ctx_t ctx, const spd_grid_desc* d, spd_grid_t* out)
         }
         SPD_HIP(hipSetDevice(ctx->c.device));
         SPD_HIP(hipMalloc((void**)&g.d_tables, sizeof(double)*9*dg.nblk));
+        // the same table read backwards: ghost region e of block D is fed by
+        // S = nbr[D][e], so S pushes its cells of that region to D
+        std::vector<int> push(27*dg.nblk, -1);
+        for (int D = 0; D < dg.nblk; ++D)
+            for (int e = 0; e < 27; ++e)
+                if (nbr[27*D + e] >= 0) push[27*nbr[27*D + e] + e] = D;
         SPD_HIP(hipMalloc((void**)&g.d_nbr, sizeof(int)*27*dg.nblk));
+        SPD_HIP(hipMalloc((void**)&g.d_push, sizeof(int)*27*dg.nblk));
         SPD_HIP(hipMalloc((void**)&g.d_start, sizeof(int)*dg.nblk));
         SPD_HIP(hipMemcpy(g.d_tables, g.h_dx.data(), sizeof(double)*3*dg.nblk, hipMemcpyHostToDevice));
         SPD_HIP(hipMemcpy(g.d_tables + 3*dg.nblk, g.h_inv_dx.data(), sizeof(double)*3*dg.nblk, hipMemcpyHostToDevice));
         SPD_HIP(hipMemcpy(g.d_tables + 6*dg.nblk, g.h_bmin.data(), sizeof(double)*3*dg.nblk, hipMemcpyHostToDevice));
         SPD_HIP(hipMemcpy(g.d_nbr, nbr.data(), sizeof(int)*27*dg.nblk, hipMemcpyHostToDevice));
+        SPD_HIP(hipMemcpy(g.d_push, push.data(), sizeof(int)*27*dg.nblk, hipMemcpyHostToDevice));
         const std::vector<int> complete(dg.nblk, dg.nc[2]);
         SPD_HIP(hipMemcpy(g.d_start, complete.data(), sizeof(int)*dg.nblk, hipMemcpyHostToDevice));
         dg.dx = g.d_tables;
@@ -250,6 +260,7 @@ int spd_grid_destroy(spd_grid_t grid)
         SPD_HIP(hipStreamSynchronize(grid->g.ctx->stream));
         SPD_HIP(hipFree(grid->g.d_tables));
         SPD_HIP(hipFree(grid->g.d_nbr));
+        SPD_HIP(hipFree(grid->g.d_push));
         SPD_HIP(hipFree(grid->g.d_start));
         if (grid->g.d_stage) SPD_HIP(hipFree(grid->g.d_stage));
         delete grid;

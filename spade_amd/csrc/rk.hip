@@ -51,8 +51,42 @@ __device__ inline void c2p(double gamma, double R, const double c[5], double q[5
 
 struct Coefs { const double* r[3]; double c[3]; int n; };
 
+// The exchange of the stage result, fused: interior cell (i, j, k) of block lb
+// is written into every ghost cell that holds it in another block.  Ghost
+// region e of block D is fed by block nbr[D][e] (grid/exchange_pattern.h:
+// 382-424); push[27 lb + e] = D names the block that region of lb's cells
+// goes to, at (i + ex nc0, j + ey nc1, k + ez nc2).  A cell within ng of the
+// low face feeds regions with e = +1 on that axis, within ng of the high face
+// regions with e = -1.  The values are the ones just stored, so the ghosts
+// equal what the exchange would copy.
+__device__ inline void push_ghosts(const DevGeom& G, const int* __restrict__ push, double* __restrict__ q,
+                                   const CellIt& c, const double qq[5])
+{
+    const int ng = G.ng;
+    const bool lo0 = c.i < ng, lo1 = c.j < ng, lo2 = c.k < ng;
+    const bool hi0 = c.i >= G.nc[0] - ng, hi1 = c.j >= G.nc[1] - ng, hi2 = c.k >= G.nc[2] - ng;
+    if (!(lo0 | lo1 | lo2 | hi0 | hi1 | hi2)) return;
+    for (int ez = -1; ez <= 1; ++ez)
+    {
+        if ((ez == 1 && !lo2) || (ez == -1 && !hi2)) continue;
+        for (int ey = -1; ey <= 1; ++ey)
+        {
+            if ((ey == 1 && !lo1) || (ey == -1 && !hi1)) continue;
+            for (int ex = -1; ex <= 1; ++ex)
+            {
+                if ((ex == 1 && !lo0) || (ex == -1 && !hi0)) continue;
+                if (ex == 0 && ey == 0 && ez == 0) continue;
+                const int D = push[27*c.lb + (ex + 1) + 3*(ey + 1) + 9*(ez + 1)];
+                if (D < 0) continue;
+                for (int v = 0; v < 5; ++v)
+                    q[cell_index(G, D, v, c.i + ex*G.nc[0], c.j + ey*G.nc[1], c.k + ez*G.nc[2])] = qq[v];
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256)
-k_rk_transform(DevGeom G, double* __restrict__ q, Coefs C, double gamma, double R)
+k_rk_transform(DevGeom G, double* __restrict__ q, Coefs C, double gamma, double R, const int* __restrict__ push)
 {
     CellIt ci;
     if (!interior_cell(G, ci)) return;
@@ -70,10 +104,12 @@ k_rk_transform(DevGeom G, double* __restrict__ q, Coefs C, double gamma, double 
     }
     c2p(gamma, R, cons, qq);
     for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
+    push_ghosts(G, push, q, ci, qq);
 }
 
 __global__ void __launch_bounds__(256)
-k_rk_opt_s0(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, double dt, double gamma, double R)
+k_rk_opt_s0(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, double dt, double gamma, double R,
+            const int* __restrict__ push)
 {
     CellIt ci;
     if (!interior_cell(G, ci)) return;
@@ -83,10 +119,12 @@ k_rk_opt_s0(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, do
     for (int v = 0; v < 5; ++v) c[v] += dt*r0[idx[v]];
     c2p(gamma, R, c, qq);
     for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
+    push_ghosts(G, push, q, ci, qq);
 }
 
 __global__ void __launch_bounds__(256)
-k_rk_opt_s1(DevGeom G, double* __restrict__ q, double* __restrict__ r0, const double* __restrict__ r1, double dt, double gamma, double R)
+k_rk_opt_s1(DevGeom G, double* __restrict__ q, double* __restrict__ r0, const double* __restrict__ r1, double dt, double gamma, double R,
+            const int* __restrict__ push)
 {
     CellIt ci;
     if (!interior_cell(G, ci)) return;
@@ -99,10 +137,12 @@ k_rk_opt_s1(DevGeom G, double* __restrict__ q, double* __restrict__ r0, const do
     for (int v = 0; v < 5; ++v) c[v] -= dt*old[v];
     c2p(gamma, R, c, qq);
     for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
+    push_ghosts(G, push, q, ci, qq);
 }
 
 __global__ void __launch_bounds__(256)
-k_rk_opt_s2(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, const double* __restrict__ r1, double dt, double gamma, double R)
+k_rk_opt_s2(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, const double* __restrict__ r1, double dt, double gamma, double R,
+            const int* __restrict__ push)
 {
     CellIt ci;
     if (!interior_cell(G, ci)) return;
@@ -114,6 +154,7 @@ k_rk_opt_s2(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, co
     for (int v = 0; v < 5; ++v) c[v] += two_thirds_dt*r1[idx[v]];
     c2p(gamma, R, c, qq);
     for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
+    push_ghosts(G, push, q, ci, qq);
 }
 
 inline dim3 interior_blocks(const DevGeom& G)
@@ -130,25 +171,25 @@ void launch_rk_transform_stage(const Grid& g, double* q, const double* const* r,
     Coefs C{};
     C.n = nr;
     for (int j = 0; j < nr; ++j) { C.r[j] = r[j]; C.c[j] = coef[j]; }
-    hipLaunchKernelGGL(k_rk_transform, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, C, gamma, R);
+    hipLaunchKernelGGL(k_rk_transform, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, C, gamma, R, g.d_push);
     SPD_LAUNCH_CHECK();
 }
 
 void launch_rk_opt_s0(const Grid& g, double* q, const double* r0, double dt, double gamma, double R, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rk_opt_s0, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, r0, dt, gamma, R);
+    hipLaunchKernelGGL(k_rk_opt_s0, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, r0, dt, gamma, R, g.d_push);
     SPD_LAUNCH_CHECK();
 }
 
 void launch_rk_opt_s1(const Grid& g, double* q, double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rk_opt_s1, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, r0, r1, dt, gamma, R);
+    hipLaunchKernelGGL(k_rk_opt_s1, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, r0, r1, dt, gamma, R, g.d_push);
     SPD_LAUNCH_CHECK();
 }
 
 void launch_rk_opt_s2(const Grid& g, double* q, const double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_rk_opt_s2, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, r0, r1, dt, gamma, R);
+    hipLaunchKernelGGL(k_rk_opt_s2, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, r0, r1, dt, gamma, R, g.d_push);
     SPD_LAUNCH_CHECK();
 }
 

@@ -58,7 +58,8 @@ struct Grid
     spd_grid_desc desc{};
     DevGeom geom{};
     double* d_tables = nullptr;   // dx, inv_dx, bmin
-    int* d_nbr = nullptr;         // [26*lb + n] source block for ghost region n, -1 = not exchanged
+    int* d_nbr = nullptr;         // [27*lb + e] source block for ghost region e, -1 = not exchanged
+    int* d_push = nullptr;        // [27*lb + e] block whose ghost region e block lb feeds, -1 = none
     int* d_start = nullptr;       // [lb] first plane the LDS-staged flux kernel left to the fix-up kernel
     double* d_stage = nullptr;    // host-layout staging for upload / download (stage_blocks blocks), lazily allocated
     int stage_blocks = 0;
@@ -106,6 +107,8 @@ void launch_fill(const Grid& g, double* a, double value, hipStream_t s);
 void launch_aos_to_soa(const Grid& g, const double* stage, double* a, int lb0, int nb, hipStream_t s);
 void launch_soa_to_aos(const Grid& g, const double* a, double* stage, int lb0, int nb, hipStream_t s);
 // RK stage: q <- c2p(p2c(q) + sum_j coef[j] * r_j) on interior cells
+// The stage kernels also write every updated cell into the ghost cells of the
+// blocks it feeds (d_push): the exchange of the stage result, fused.
 void launch_rk_transform_stage(const Grid& g, double* q, const double* const* r, const double* coef, int nr,
                                double gamma, double R, hipStream_t s);
 void launch_rk_opt_s0(const Grid& g, double* q, const double* r0, double dt, double gamma, double R, hipStream_t s);

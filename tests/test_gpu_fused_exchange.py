@@ -1,0 +1,84 @@
+"""The integrator's stage kernels write the exchanged ghost cells themselves
+(the exchange of each stage result is fused into the stage update).  After
+any number of steps the ghosts must equal, bit for bit, what the stand-alone
+exchange (grid/exchange_pattern.h:382-424) followed by the wall fill copies
+from the same interior -- on every stored cell, for:
+
+* a periodic multi-block grid (both SSP-RK3 forms);
+* a grid with isothermal walls on the non-periodic y axis (edges and corners
+  that cross a wall are not exchanged; the wall fill runs after);
+* one block along an axis (the block is its own neighbour) and blocks no wider
+  than the ghost layers (a cell feeds the low and the high ghosts at once);
+* 16^3 blocks (the LDS-staged flux kernel), several steps.
+"""
+import numpy as np
+import pytest
+
+import golden_io as gi
+from cases import FLUX_CASES, GAMMA, R, P
+from gpu_helpers import RawFlux
+
+pytestmark = pytest.mark.gpu
+
+
+def _sa():
+    import spade_amd as sa
+    sa.load()
+    return sa
+
+
+def _state(sa, g, seed):
+    """A smooth state with a little noise (T near 1) and current ghosts."""
+    rng = np.random.default_rng(seed)
+    host = np.zeros(g.host_shape)
+    ng = g.num_exch
+    inner = host[:, ng:-ng, ng:-ng, ng:-ng]
+    inner[..., 0] = P["p0"] * (1.0 + 0.01 * rng.standard_normal(inner.shape[:-1]))
+    inner[..., 1] = 1.0 + 0.01 * rng.standard_normal(inner.shape[:-1])
+    inner[..., 2:] = 0.1 * rng.standard_normal(inner.shape[:-1] + (3,))
+    q = sa.GridArray(g).upload(host)
+    sa.create_exchange(q).exchange(q)
+    return q
+
+
+def _check(sa, g, scheme, steps, walls=(), seed=1, dt=1e-4):
+    q = _state(sa, g, seed)
+    for f in walls:
+        sa.boundary_fill_wall(q, f, P["t_wall"])
+    it = sa.Integrator(q, RawFlux(FLUX_CASES["keep2_visc_pow"], GAMMA, R), dt, scheme=scheme,
+                       walls=walls, t_wall=P["t_wall"])
+    it.advance(steps)
+    fused = q.download()
+    sa.create_exchange(q).exchange(q)
+    for f in walls:
+        sa.boundary_fill_wall(q, f, P["t_wall"])
+    pulled = q.download()
+    assert np.all(np.isfinite(fused))
+    assert np.array_equal(fused, pulled), np.argwhere(fused != pulled)[:5]
+
+
+@pytest.mark.parametrize("scheme", ["ssprk3", "ssprk3_opt"])
+def test_periodic(scheme):
+    sa = _sa()
+    g = sa.CartesianGrid((4, 8, 4), (2, 3, 2), ((0.0, 1.0), (0.0, 2.0), (0.0, 1.5)), num_exch=2)
+    _check(sa, g, getattr(sa, scheme), 3)
+
+
+def test_walls():
+    sa = _sa()
+    g = sa.CartesianGrid((4, 8, 4), (2, 3, 2), ((0.0, 1.0), (0.0, 2.0), (0.0, 1.5)), num_exch=2,
+                         periodic=(True, False, True))
+    _check(sa, g, sa.ssprk3, 3, walls=(2, 3))
+
+
+def test_self_neighbour_and_narrow_blocks():
+    sa = _sa()
+    g = sa.CartesianGrid((2, 3, 4), (1, 2, 3), ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), num_exch=2)
+    _check(sa, g, sa.ssprk3, 2)
+    _check(sa, g, sa.ssprk3_opt, 2, seed=2)
+
+
+def test_fast_kernel_blocks():
+    sa = _sa()
+    g = sa.CartesianGrid((16, 16, 16), (2, 2, 2), ((0.0, 1.0),) * 3, num_exch=2)
+    _check(sa, g, sa.ssprk3, 4)
