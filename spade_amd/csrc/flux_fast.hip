@@ -321,7 +321,7 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     const long long sv = G.sv;
     const int N1 = G.N[1], ng = G.ng;
     // plane (k) strides of a core column and of an x-halo column
-    const long long plane = (long long)G.nc[0]*N1, xplane = (long long)2*N1;
+    const long long plane = (long long)G.nc[0]*N1, xplane = (long long)N1;
 
     // halo cell of this thread: waves 0 and 1 own the 64 edge cells and the
     // 4 corners of the 18x18 plane (waves 2 and 3 evaluate the block-boundary faces)

@@ -59,6 +59,15 @@ struct Coefs { const double* r[3]; double c[3]; int n; };
 // low face feeds regions with e = +1 on that axis, within ng of the high face
 // regions with e = -1.  The values are the ones just stored, so the ghosts
 // equal what the exchange would copy.
+__device__ inline void push_one(const DevGeom& G, const int* __restrict__ push, double* __restrict__ q,
+                                const CellIt& c, const double qq[5], int ex, int ey, int ez)
+{
+    const int D = push[27*c.lb + (ex + 1) + 3*(ey + 1) + 9*(ez + 1)];
+    if (D < 0) return;
+    for (int v = 0; v < 5; ++v)
+        q[cell_index(G, D, v, c.i + ex*G.nc[0], c.j + ey*G.nc[1], c.k + ez*G.nc[2])] = qq[v];
+}
+
 __device__ inline void push_ghosts(const DevGeom& G, const int* __restrict__ push, double* __restrict__ q,
                                    const CellIt& c, const double qq[5])
 {
@@ -66,6 +75,20 @@ __device__ inline void push_ghosts(const DevGeom& G, const int* __restrict__ pus
     const bool lo0 = c.i < ng, lo1 = c.j < ng, lo2 = c.k < ng;
     const bool hi0 = c.i >= G.nc[0] - ng, hi1 = c.j >= G.nc[1] - ng, hi2 = c.k >= G.nc[2] - ng;
     if (!(lo0 | lo1 | lo2 | hi0 | hi1 | hi2)) return;
+    if (G.nc[0] >= 2*ng && G.nc[1] >= 2*ng && G.nc[2] >= 2*ng)
+    {
+        // at most one direction per axis: the cell's own face set, 1 to 7 regions
+        const int ex = lo0 ? 1 : (hi0 ? -1 : 0), ey = lo1 ? 1 : (hi1 ? -1 : 0), ez = lo2 ? 1 : (hi2 ? -1 : 0);
+        if (ex)             push_one(G, push, q, c, qq, ex, 0, 0);
+        if (ey)             push_one(G, push, q, c, qq, 0, ey, 0);
+        if (ez)             push_one(G, push, q, c, qq, 0, 0, ez);
+        if (ex && ey)       push_one(G, push, q, c, qq, ex, ey, 0);
+        if (ex && ez)       push_one(G, push, q, c, qq, ex, 0, ez);
+        if (ey && ez)       push_one(G, push, q, c, qq, 0, ey, ez);
+        if (ex && ey && ez) push_one(G, push, q, c, qq, ex, ey, ez);
+        return;
+    }
+    // blocks narrower than two ghost layers: a cell can feed both sides of an axis
     for (int ez = -1; ez <= 1; ++ez)
     {
         if ((ez == 1 && !lo2) || (ez == -1 && !hi2)) continue;
@@ -76,10 +99,7 @@ __device__ inline void push_ghosts(const DevGeom& G, const int* __restrict__ pus
             {
                 if ((ex == 1 && !lo0) || (ex == -1 && !hi0)) continue;
                 if (ex == 0 && ey == 0 && ez == 0) continue;
-                const int D = push[27*c.lb + (ex + 1) + 3*(ey + 1) + 9*(ez + 1)];
-                if (D < 0) continue;
-                for (int v = 0; v < 5; ++v)
-                    q[cell_index(G, D, v, c.i + ex*G.nc[0], c.j + ey*G.nc[1], c.k + ez*G.nc[2])] = qq[v];
+                push_one(G, push, q, c, qq, ex, ey, ez);
             }
         }
     }

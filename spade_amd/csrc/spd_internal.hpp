@@ -121,9 +121,9 @@ double reduce_max_wavespeed(const Grid& g, const double* q, double gamma, double
 //   core:   every stored row (j, k in [-ng, n+ng)), ghost rows included, holds
 //           its nc0 interior-x cells (i = 0 .. nc0-1); rows in (j, k) order,
 //           pitch nc0 -- with nc0 = 16 a row is one 128-byte line;
-//   x-halo: at offset xh, one section per ghost layer (layer 0: i = -1 and
-//           nc0; layer 1: i = -2 and nc0+1; ...), each holding the (low, high)
-//           pair of every row in the same row order.
+//   x-halo: at offset xh, one section per ghost layer and side (layer 0 low:
+//           i = -1, layer 0 high: i = nc0, layer 1 low: i = -2, ...), each
+//           holding one cell of every row in the same row order.
 // Interior rows are whole, aligned lines (row stores that are not 64-byte
 // aligned run at a third of the aligned rate on MI355X HBM), the x-ghost
 // cells of consecutive rows are contiguous (the exchange writes whole lines,
@@ -132,8 +132,8 @@ __host__ __device__ inline long long stored_offset(const DevGeom& g, int i, int 
 {
     const long long row = (long long)(k + g.ng)*g.N[1] + (j + g.ng);
     if (i >= 0 && i < g.nc[0]) return row*g.nc[0] + i;
-    const int layer = (i < 0) ? -1 - i : i - g.nc[0];
-    return g.xh + (long long)layer*2*g.N[1]*g.N[2] + 2*row + (i < 0 ? 0 : 1);
+    const int section = (i < 0) ? 2*(-1 - i) : 2*(i - g.nc[0]) + 1;
+    return g.xh + (long long)section*g.N[1]*g.N[2] + row;
 }
 
 // Index of a stored cell (i, j, k in [-ng, n+ng)) of variable v in block lb.
