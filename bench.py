@@ -126,6 +126,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--serve-requests", type=int, default=60)
     ap.add_argument("--serve-rate", type=float, default=8.0, help="Poisson arrival rate, requests/s")
+    ap.add_argument("--serve-rate-saturated", type=float, default=64.0,
+                    help="arrival rate of the second served run (above the engine's capacity)")
     ap.add_argument("--copy-check", type=int, default=0,
                     help="after the timed loop, copy the primitive array N times (a plain device copy of "
                          "known size, used to calibrate rocprofv3 FETCH_SIZE / WRITE_SIZE)")
@@ -255,11 +257,15 @@ def chosen_form(args, log):
     # serving: Poisson arrivals, continuous batching (--serve-requests 0 leaves it out)
     if args.serve_requests <= 0:
         return out
-    srv, _ = studies.serving(args.serve_requests, args.serve_rate)
-    srv["speedup_vs_fp64_same_cases"] = srv["cell_updates_per_s"] / fp64_ens
-    out["served"] = srv
-    log(f"served {srv['cell_updates_per_s'] / 1e9:.3f} Gcell-updates/s, p50 {srv['p50_latency_s']:.3f}s "
-        f"p99 {srv['p99_latency_s']:.3f}s")
+    fp64_256 = out["fp64_single_case"]["cell_updates_per_s"]
+    for key, rate in (("served", args.serve_rate), ("served_saturated", args.serve_rate_saturated)):
+        # light load (latency) and an arrival rate above the engine's capacity (throughput)
+        srv, _ = studies.serving(args.serve_requests, rate)
+        srv["speedup_vs_fp64_same_cases"] = srv["cell_updates_per_s"] / fp64_ens
+        srv["speedup_vs_fp64_256"] = srv["cell_updates_per_s"] / fp64_256
+        out[key] = srv
+        log(f"{key} {srv['cell_updates_per_s'] / 1e9:.3f} Gcell-updates/s at {rate:g}/s, "
+            f"p50 {srv['p50_latency_s']:.3f}s p99 {srv['p99_latency_s']:.3f}s")
     return out
 
 
