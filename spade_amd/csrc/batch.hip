@@ -42,10 +42,15 @@ template <int D> __device__ inline float velf(const CellF& c) { return D == 0 ? 
 
 // FP32 face flux of cent_keep<2> + visc_lr about the reference state.
 // Outputs (f32): continuity perturbation, energy perturbation, x/y/z momentum.
+// The FP32 face arithmetic may fuse a*b+c into one FMA: this engine has no
+// bit-for-bit contract with the reference, and a face value is computed once
+// and shared by the two cells it separates, so the FP64 residual sums (kept
+// without contraction) still telescope exactly over a periodic case.
 template <int D, int LAW>
 __device__ inline void face_f32(const CaseParams& P, const CellF& L, const CellF& R, const float g[3][3], float gT,
                                 float F[5])
 {
+    #pragma clang fp contract(fast)
     const float unL = velf<D>(L), unR = velf<D>(R);
     const float U   = 0.5f*(unL + unR);
     const float Fcp = 0.5f*(L.rho + R.rho)*U;                // rho'-part of the mass flux
@@ -76,7 +81,11 @@ __device__ inline void face_f32(const CaseParams& P, const CellF& L, const CellF
     F[4] = Fm[2] - t[2];
 }
 
-__device__ inline float tangf(float a, float b, float c, float d, float it) { return 0.25f*it*((a - b) + (c - d)); }
+__device__ inline float tangf(float a, float b, float c, float d, float it)
+{
+    #pragma clang fp contract(fast)
+    return 0.25f*it*((a - b) + (c - d));
+}
 
 // slot layout of the f32 velocity ring: s[pos] = u, s[HPB + pos] = v, s[2*HPB + pos] = w
 __device__ inline void gradsf_x(const float* sm, const float* s0, const float* sp, int pL, int pR,
