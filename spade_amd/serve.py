@@ -8,8 +8,11 @@ Reynolds number and step count) arrive over time.  The loop
     later, smaller request pass a head request that does not fit yet
     (backfill), so the device stays busy;
   * keeps every admitted case resident on the device and advances all of them
-    together, a few SSP-RK3 steps per round, so newly admitted cases join the
-    running ones and finished cases leave between rounds;
+    together, a few SSP-RK3 steps per round (at most max_chunk while requests
+    are still due to arrive, so a new one waits at most that long; once all
+    have arrived, as many as the shortest remaining case allows), so newly
+    admitted cases join the running ones and finished cases leave between
+    rounds;
   * retires a case when it has done its step count: its final conserved
     fields and its per-step kinetic-energy / enstrophy history are copied back
     and its slots are freed for the queue.
@@ -88,7 +91,9 @@ class Server:
                 if queue:
                     raise RuntimeError(f"request {queue[0].rid} can never fit in the engine's pool")
                 break
-            k = min(self.max_chunk, min(a[1] for a in active.values()))
+            k = min(a[1] for a in active.values())
+            if pending:
+                k = min(k, self.max_chunk)
             self.engine.step(k)
             self.rounds += 1
             done_at = self.clock() - t0
