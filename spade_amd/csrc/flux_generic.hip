@@ -12,6 +12,7 @@
 // kernel in flux_fast.hip.
 #include "spd_internal.hpp"
 #include <cmath>
+#include <algorithm>
 
 namespace spd {
 namespace {
@@ -564,21 +565,25 @@ __global__ void __launch_bounds__(256)
 k_flux_fixup(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, spd_flux_desc F, int increment,
              const int* __restrict__ start)
 {
-    const int lb = blockIdx.x;
-    const int nz = G.nc[2];
-    int s = start[lb];
-    if (s >= nz) return;
-    if (s < 0) s = 0;
     __shared__ DevGeom sG;                // see k_flux_div_generic
     __shared__ spd_flux_desc sF;
     if (threadIdx.x == 0) { sG = G; sF = F; }
     __syncthreads();
+    const int nz = G.nc[2];
     const int per_plane = G.nc[0]*G.nc[1];
-    const int n = per_plane*(nz - s);
-    for (int c = threadIdx.x; c < n; c += blockDim.x)
+    // a grid of at most 1024 workgroups walks the blocks: when no block needs
+    // completing (the usual case) the launch costs a few microseconds
+    for (int lb = blockIdx.x; lb < G.nblk; lb += gridDim.x)
     {
-        const int i = c % G.nc[0], j = (c / G.nc[0]) % G.nc[1], k = s + c/per_plane;
-        generic_cell(sG, prims, rhs, sF, increment, i, j, k, lb);
+        int s = start[lb];
+        if (s >= nz) continue;
+        if (s < 0) s = 0;
+        const int n = per_plane*(nz - s);
+        for (int c = threadIdx.x; c < n; c += blockDim.x)
+        {
+            const int i = c % G.nc[0], j = (c / G.nc[0]) % G.nc[1], k = s + c/per_plane;
+            generic_cell(sG, prims, rhs, sF, increment, i, j, k, lb);
+        }
     }
 }
 
@@ -596,8 +601,8 @@ void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, co
 void launch_flux_div_fixup(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f, bool increment,
                            hipStream_t s)
 {
-    hipLaunchKernelGGL(k_flux_fixup, dim3(g.geom.nblk), dim3(256), 0, s, g.geom, prims, rhs, f, increment ? 1 : 0,
-                       g.d_start);
+    hipLaunchKernelGGL(k_flux_fixup, dim3(std::min(g.geom.nblk, 1024)), dim3(256), 0, s, g.geom, prims, rhs, f,
+                       increment ? 1 : 0, g.d_start);
     SPD_LAUNCH_CHECK();
 }
 
