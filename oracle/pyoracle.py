@@ -58,6 +58,8 @@ def _load():
     dp = C.POINTER(C.c_double)
     lib.ora_array_size.argtypes = [C.POINTER(Grid)]
     lib.ora_flux_div.argtypes = [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Flux), dp, dp, C.c_int]
+    lib.ora_flux_div_blocks.argtypes = [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Flux), dp, dp, C.c_int,
+                                        C.POINTER(C.c_int), C.c_int]
     lib.ora_exchange.argtypes = [C.POINTER(Grid), dp]
     lib.ora_boundary_fill_wall.argtypes = [C.POINTER(Grid), dp, C.c_int, C.c_double]
     lib.ora_prim_to_cons.argtypes = [C.POINTER(Gas), dp, dp, C.c_long]
@@ -126,6 +128,18 @@ def flux_div(g, gas, flux, prim, rhs=None, increment=False):
     rc = lib().ora_flux_div(C.byref(g), C.byref(gas), C.byref(flux), _ptr(prim), _ptr(rhs), int(increment))
     if rc != 0:
         raise RuntimeError("ora_flux_div failed")
+    return rhs
+
+
+def flux_div_blocks(g, gas, flux, prim, blocks, rhs=None, increment=False):
+    """flux_div on the interior cells of the listed blocks only."""
+    if rhs is None:
+        rhs = np.zeros(shape_of(g))
+    bl = np.ascontiguousarray(blocks, dtype=np.int32)
+    rc = lib().ora_flux_div_blocks(C.byref(g), C.byref(gas), C.byref(flux), _ptr(prim), _ptr(rhs), int(increment),
+                                   bl.ctypes.data_as(C.POINTER(C.c_int)), len(bl))
+    if rc != 0:
+        raise RuntimeError("ora_flux_div_blocks failed")
     return rhs
 
 

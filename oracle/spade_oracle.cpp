@@ -657,6 +657,31 @@ template <typename F> void for_interior(const Geo& G, F&& fn)
                 for (int i = 0; i < G.g.nc[0]; ++i) fn(i, j, k, lb);
 }
 
+// rhs of interior cell (i, j, k) of block lb, flux_div_basic's loop body
+// (flux_div_basic.h:47-72), accumulating onto the current rhs value
+void flux_div_cell(const Geo& G, const Ctx& C, const ora_flux& fl, double* rhs, int i, int j, int k, int lb)
+{
+    const double jac = cell_jac(G, lb, i, j, k);
+    double* r = rhs + G.cell(i, j, k, lb);
+    double relem[5] = {r[0], r[1], r[2], r[3], r[4]};
+    for (int idir = 0; idir < 3; ++idir)
+    {
+        const double inv_dx = 1.0/G.dx[3*lb + idir];
+        for (int pm = 0; pm < 2; ++pm)
+        {
+            Face f{idir, i, j, k, lb};
+            if (pm == 1) { if (idir == 0) f.i++; else if (idir == 1) f.j++; else f.k++; }
+            double acc[5] = {0, 0, 0, 0, 0};
+            double fx[5];
+            eval_flux(C, f, fl, fx);
+            for (int n = 0; n < 5; ++n) acc[n] += fx[n];
+            const double s = jac*(1.0 - 2.0*pm)*inv_dx;
+            for (int n = 0; n < 5; ++n) { acc[n] *= s; relem[n] += acc[n]; }
+        }
+    }
+    for (int n = 0; n < 5; ++n) r[n] = relem[n];
+}
+
 void flux_div(const Geo& G, const double* prim, double* rhs, const ora_flux& fl, const ora_gas& gas, int increment)
 {
     if (!increment)
@@ -665,28 +690,7 @@ void flux_div(const Geo& G, const double* prim, double* rhs, const ora_flux& fl,
         std::fill(rhs, rhs + tot, 0.0);
     }
     const Ctx C{G, prim, gas};
-    for_interior(G, [&](int i, int j, int k, int lb)
-    {
-        const double jac = cell_jac(G, lb, i, j, k);
-        double* r = rhs + G.cell(i, j, k, lb);
-        double relem[5] = {r[0], r[1], r[2], r[3], r[4]};
-        for (int idir = 0; idir < 3; ++idir)
-        {
-            const double inv_dx = 1.0/G.dx[3*lb + idir];
-            for (int pm = 0; pm < 2; ++pm)
-            {
-                Face f{idir, i, j, k, lb};
-                if (pm == 1) { if (idir == 0) f.i++; else if (idir == 1) f.j++; else f.k++; }
-                double acc[5] = {0, 0, 0, 0, 0};
-                double fx[5];
-                eval_flux(C, f, fl, fx);
-                for (int n = 0; n < 5; ++n) acc[n] += fx[n];
-                const double s = jac*(1.0 - 2.0*pm)*inv_dx;
-                for (int n = 0; n < 5; ++n) { acc[n] *= s; relem[n] += acc[n]; }
-            }
-        }
-        for (int n = 0; n < 5; ++n) r[n] = relem[n];
-    });
+    for_interior(G, [&](int i, int j, int k, int lb) { flux_div_cell(G, C, fl, rhs, i, j, k, lb); });
 }
 
 // 26-neighbour exchange (exchange_pattern.h:382-424, get_transaction.h:55-85)
@@ -749,6 +753,27 @@ int ora_flux_div(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, cons
     if (!g || !gas || !fl || !prim || !rhs || fl->nterms < 1 || fl->nterms > ORA_MAX_TERMS) return -1;
     const Geo G(g);
     flux_div(G, prim, rhs, *fl, *gas, increment);
+    return 0;
+}
+
+int ora_flux_div_blocks(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, const double* prim, double* rhs,
+                        int increment, const int* blocks, int nblocks)
+{
+    if (!g || !gas || !fl || !prim || !rhs || !blocks || fl->nterms < 1 || fl->nterms > ORA_MAX_TERMS) return -1;
+    const Geo G(g);
+    for (int b = 0; b < nblocks; ++b) if (blocks[b] < 0 || blocks[b] >= G.nblk) return -1;
+    const Ctx C{G, prim, *gas};
+    for (int b = 0; b < nblocks; ++b)
+    {
+        const int lb = blocks[b];
+        for (int k = 0; k < G.g.nc[2]; ++k)
+            for (int j = 0; j < G.g.nc[1]; ++j)
+                for (int i = 0; i < G.g.nc[0]; ++i)
+                {
+                    if (!increment) { double* r = rhs + G.cell(i, j, k, lb); for (int n = 0; n < 5; ++n) r[n] = 0.0; }
+                    flux_div_cell(G, C, *fl, rhs, i, j, k, lb);
+                }
+    }
     return 0;
 }
 

@@ -56,6 +56,10 @@ typedef struct { int wall_mask; double t_wall; } ora_bc;
 int ora_array_size(const ora_grid* g);
 int ora_flux_div(const ora_grid* g, const ora_gas* gas, const ora_flux* fl,
                  const double* prim, double* rhs, int increment);
+/* flux_div on the interior cells of the listed blocks only (the others are
+ * left untouched): the same per-cell arithmetic as ora_flux_div */
+int ora_flux_div_blocks(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, const double* prim, double* rhs,
+                        int increment, const int* blocks, int nblocks);
 int ora_exchange(const ora_grid* g, double* a);
 int ora_boundary_fill_wall(const ora_grid* g, double* a, int face, double t_wall);
 int ora_prim_to_cons(const ora_gas* gas, const double* prim, double* cons, long ncell);
