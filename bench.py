@@ -216,7 +216,7 @@ def main():
         for _ in range(args.copy_check):
             rhs.copy_from(q)
         g.ctx.sync()
-        # the runtime copies the whole device allocation (padded rows)
+        # the runtime copies the whole device allocation
         line["copy_check_bytes"] = 8 * g.device_size
     del q, rhs
     if not args.no_chosen_form:

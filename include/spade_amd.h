@@ -78,8 +78,9 @@ int spd_array_fill(spd_array_t arr, double value);
 int spd_array_copy(spd_array_t dst, spd_array_t src);
 /* Number of doubles of the reference-layout host image of an array. */
 long spd_array_host_size(spd_grid_t grid);
-/* Number of doubles one array occupies on the device (rows padded to a
- * 128-byte pitch, blocks to 256 bytes); no counterpart in the reference. */
+/* Number of doubles one array occupies on the device: per block and
+ * variable the stored cells (interior-x rows, then the x-ghost sections),
+ * blocks padded to 256 bytes; no counterpart in the reference. */
 long spd_array_device_size(spd_grid_t grid);
 
 /* ---- flux functors (composed as F0 + (F1 + ...), omni/compose.h:7-45) ---- */

@@ -147,7 +147,7 @@ class CartesianGrid:
 
     @property
     def device_size(self):
-        """Doubles one array of this grid occupies on the device (padded rows)."""
+        """Doubles one array of this grid occupies on the device (blocks padded to 256 bytes)."""
         return int(load().spd_array_device_size(self._h))
 
     @property

@@ -5,14 +5,17 @@
 // One workgroup (256 threads, thread (i,j)) owns one block and marches over k.
 // Every face flux is evaluated once: thread (i,j) evaluates the low x-face
 // i-1/2, the low y-face j-1/2 and the high z-face k+1/2 of its cell; the 16
-// block-boundary x-faces and y-faces of a plane are evaluated by 16 lanes of
-// waves 0 and 1.  Face products P = F * (1/dx) are exchanged through LDS
-// (y-faces, boundary x-faces) and ds_bpermute (x-faces); the z-face product
-// stays in registers for the next plane.
+// block-boundary x-faces and the 16 y-faces of a plane are evaluated by 16
+// lanes of waves 2 and 3.  In-plane face products P = F * (1/dx) are exchanged
+// through LDS; the z-face product stays in registers for the next plane.
 //
 // LDS holds, for the three planes k-1, k, k+1, the velocities (u,v,w) on the
-// 18x18 plane with one halo layer (ring of three slots), and for plane k the
-// p, T, rho, e needed as left/right states of the in-plane faces.
+// 18x18 plane with one halo layer (ring of three slots), and for planes k and
+// k+1 the p, T, rho, e needed as left/right states of the faces.
+//
+// Power law: x^n comes from a binomial series valid near x = 1.  A block
+// with a face temperature outside its range stops at that plane, and
+// k_flux_fixup (flux_generic.hip) completes it with the library pow.
 //
 // Arithmetic follows the reference's expression order exactly
 // (navier-stokes/convective.h:117-184, 212-245; viscous.h:39-112;
