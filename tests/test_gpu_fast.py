@@ -49,10 +49,9 @@ def _state(nb, nc, ng, seed):
     return q, L
 
 
-def _run(nb, nc, terms, seed, periodic=(True, True, True), walls=(), increment=False):
+def _run(nb, nc, terms, seed, periodic=(True, True, True), walls=(), increment=False, ng=2):
     import spade_amd as sa
     sa.load()
-    ng = 2
     host, L = _state(nb, nc, ng, seed)
     g = sa.CartesianGrid(nc, nb, ((0.0, L),) * 3, num_exch=ng, periodic=periodic)
     q = sa.GridArray(g).upload(host)
@@ -94,6 +93,26 @@ def test_fast_increment_cubic_blocks():
 
 def test_fast_thin_blocks():
     got, ref = _run((1, 2, 3), (16, 16, 2), COMBOS["keep2_visc_const"], seed=5)
+    assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("nz, ng", [(1, 1), (3, 2), (5, 2)])
+def test_fast_odd_and_single_plane_blocks(nz, ng):
+    """The k-march takes planes in pairs: an odd number of planes ends on the
+    first plane of a pair, and a single plane (one ghost layer: blocks are at
+    least as wide as the ghost layers) is the prologue plus one step."""
+    got, ref = _run((2, 1, 3), (16, 16, nz), COMBOS["keep2_pdiss_ducros_visc_pow"], seed=20 + nz,
+                    increment=(nz == 3), ng=ng)
+    err = gi.rel_maxnorm(got, ref)
+    print("nz", nz, err, flush=True)
+    assert np.all(err <= 1e-12), err
+
+
+def test_fast_one_ghost_layer():
+    """cent_keep<2>, pressure_diss_lr and visc_lr reach one cell across a
+    face, so one ghost layer suffices: the x-halo then holds one section per
+    side."""
+    got, ref = _run((2, 2, 2), (16, 16, 3), COMBOS["keep2_pdiss_const_visc_const"], seed=31, ng=1)
     assert np.array_equal(got, ref)
 
 
