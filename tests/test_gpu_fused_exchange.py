@@ -9,6 +9,7 @@ from the same interior -- on every stored cell, for:
   that cross a wall are not exchanged; the wall fill runs after);
 * one block along an axis (the block is its own neighbour) and blocks no wider
   than the ghost layers (a cell feeds the low and the high ghosts at once);
+* one ghost layer (walls, and 16x16 blocks through the LDS-staged flux kernel);
 * 16^3 blocks (the LDS-staged flux kernel), several steps.
 """
 import numpy as np
@@ -76,6 +77,15 @@ def test_self_neighbour_and_narrow_blocks():
     g = sa.CartesianGrid((2, 3, 4), (1, 2, 3), ((0.0, 1.0), (0.0, 1.0), (0.0, 1.0)), num_exch=2)
     _check(sa, g, sa.ssprk3, 2)
     _check(sa, g, sa.ssprk3_opt, 2, seed=2)
+
+
+def test_one_ghost_layer():
+    sa = _sa()
+    g = sa.CartesianGrid((4, 8, 3), (2, 3, 2), ((0.0, 1.0), (0.0, 2.0), (0.0, 1.5)), num_exch=1,
+                         periodic=(True, False, True))
+    _check(sa, g, sa.ssprk3, 3, walls=(2, 3))
+    g = sa.CartesianGrid((16, 16, 3), (2, 1, 2), ((0.0, 1.0),) * 3, num_exch=1)
+    _check(sa, g, sa.ssprk3_opt, 2, seed=4)
 
 
 def test_fast_kernel_blocks():
