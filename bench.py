@@ -89,6 +89,37 @@ def cpu_baseline(sample_blocks=4, reps=5):
             "cpu": _cpu_model()}
 
 
+def cpu_baseline_all_cores(sample_blocks=4, reps=5, max_procs=16):
+    """The compiled reference's CPU flux_div on every core at once: one
+    single-rank process per core, each on its own sample of blocks, as the
+    reference runs one MPI rank per core over its share of the blocks
+    (partition.h:27-84).  The aggregate is the sum of the processes' rates,
+    all measured while every process runs.  None without the compiled
+    reference."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
+    if not os.path.exists(exe):
+        return None
+    n = max(1, min(max_procs, len(os.sched_getaffinity(0))))
+    procs = [subprocess.Popen([exe, "--bench", str(sample_blocks), str(reps)], stdout=subprocess.PIPE,
+                              stderr=subprocess.DEVNULL, text=True) for _ in range(n)]
+    rates = []
+    try:
+        for p in procs:
+            out, _ = p.communicate(timeout=300)
+            if p.returncode != 0:
+                raise RuntimeError(f"ref_driver exited with {p.returncode}")
+            rates.append(json.loads([l for l in out.splitlines() if l.startswith("{")][-1])["mcell_rhs_per_s"])
+    except Exception as e:  # pragma: no cover - reported, the one-core figure stands alone
+        for p in procs:
+            if p.poll() is None:
+                p.kill()
+        print(f"all-core reference baseline unavailable: {e}", file=sys.stderr)
+        return None
+    return {"value": round(sum(rates), 4), "unit": "Mcell-RHS/s", "cores": n, "kind": "reference",
+            "sample": f"{n} single-rank processes at once, each TGV {sample_blocks}^3 blocks of 16^3, "
+                      f"median of {reps} each; sum of their rates"}
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -223,6 +254,10 @@ def main():
         line["chosen_form"] = chosen_form(args, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu_baseline:
         cb = cpu_baseline()
+        if cb["kind"] == "reference":
+            allc = cpu_baseline_all_cores()
+            if allc is not None:
+                cb["all_cores"] = allc
         line["cpu_baseline"] = cb
     print(json.dumps(line), flush=True)
 
