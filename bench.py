@@ -21,6 +21,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_CELL_RHS = 80.0   # read 5 x 8 B primitives + write 5 x 8 B residual
+# MI355X VALU issue: a wave64 instruction every 4 cycles per SIMD (FP64 at
+# full rate), 256 CUs x 4 SIMDs, 2.4 GHz peak engine clock
+VALU_CYCLES_PER_WAVE_INST = 4
+SIMDS = 1024
+CLOCK_GHZ = 2.4
 
 
 def tgv_interior(nb, nc, L, p0):
@@ -134,12 +139,31 @@ def _cpu_model():
 def _traffic(workload):
     """HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
     (profiles/traffic.json), or None when not measured."""
+    return _profiled(workload)
+
+
+def _profiled(key):
+    """A per-launch figure measured with rocprofv3 counters and kept in
+    profiles/traffic.json, or None when not measured."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as fh:
-            return json.load(fh).get(workload)
+            return json.load(fh).get(key)
     except (OSError, ValueError):
         return None
+
+
+def _valu_issue(workload, ms):
+    """FP64 VALU issue beside the HBM roofline: the kernel's SQ_INSTS_VALU wave
+    instructions (profiles/traffic.json) at 4 cycles each on 1024 SIMDs at the
+    2.4 GHz peak clock, against the measured time per launch."""
+    insts = _profiled(workload + "_valu_wave_insts")
+    if insts is None:
+        return None
+    issue_us = insts * VALU_CYCLES_PER_WAVE_INST / (SIMDS * CLOCK_GHZ * 1e3)
+    return {"wave_insts": insts, "issue_us": round(issue_us, 1), "frac": round(issue_us / (ms * 1e3), 4),
+            "note": "VALU issue time (wave instructions x 4 cycles / 1024 SIMDs / 2.4 GHz) over the "
+                    "measured time per flux_div"}
 
 
 def main():
@@ -230,6 +254,7 @@ def main():
                    "cells": cells},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic},
+        "valu_issue": _valu_issue(workload, ms),
     }
     # drop-in use from host memory: the reference-layout host arrays cross the
     # boundary (layout conversion + PCIe) around one exchange + flux_div
