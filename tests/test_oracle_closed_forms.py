@@ -17,9 +17,11 @@
   fields that couple the axes.  Dropping the metric factor of the tangential
   gradient components in a copy of the oracle gives O(1) errors and order ~0
   here, while freestream, conservation and the linear field still pass.
-* Second-order convergence of cent_keep<2> on the same stretched grid towards
-  the Euler divergence derived symbolically; unweighted face normals (the
-  area factor x' y' z' / x_d' left out) in a copy of the oracle give O(1)
+* Convergence of every convective flux on the same stretched grid towards
+  the Euler divergence derived symbolically, at its formal order (cent_keep<2>
+  and totani 2, cent_keep<4> 4, WENO / FWENO with linear weights 3,
+  first_order 1; measured 2.0, 4.0, 2.9-4.0, 0.94); unweighted face normals
+  (the area factor x' y' z' / x_d' left out) in a copy of the oracle give O(1)
   errors here while freestream and conservation still pass.  (For diagonal
   maps a face's normal depends only on the other axes' derivatives, which the
   face and both of its cells share, so where the metric is evaluated cannot
@@ -205,11 +207,15 @@ def test_keep2_second_order_consistency():
 
 
 def _fields(sp):
-    """Smooth primitive fields of physical (x, y, z) that couple the axes."""
+    """Smooth primitive fields of physical (x, y, z) that couple the axes.  A
+    mean flow keeps |u| > 0 everywhere: the Rusanov splitting's dissipation
+    coefficient (|u| + c) / 2 (flux_funcs.h:8-52) has a kink where the
+    velocity vanishes, and its mass and energy outputs would not converge in
+    the max norm at such a point."""
     x, y, z = sp.symbols("x y z")
-    u = [3 * sp.sin(1.5 * y + 0.3) * sp.cos(x) + 0.2 * sp.sin(z),
-         2 * sp.cos(z) * sp.sin(x + 0.5 * y),
-         sp.sin(x) * sp.cos(y) + 0.5 * sp.sin(z)]
+    u = [20 + 3 * sp.sin(1.5 * y + 0.3) * sp.cos(x) + 0.2 * sp.sin(z),
+         5 + 2 * sp.cos(z) * sp.sin(x + 0.5 * y),
+         -8 + sp.sin(x) * sp.cos(y) + 0.5 * sp.sin(z)]
     T = 300 + 5 * sp.cos(1.2 * y) * sp.sin(x) + 2 * sp.sin(z) + 1.5 * sp.cos(x)
     p = 1e5 + 500 * sp.sin(x) * sp.cos(1.3 * y) + 300 * sp.cos(z)
     return (x, y, z), p, T, u
@@ -302,18 +308,32 @@ def test_visc_second_order_on_stretched_grid(law):
     assert np.all(order[-1] > 1.8), order
 
 
-def test_keep2_second_order_on_stretched_grid():
-    """cent_keep<2> on a tanh-stretched axis converges at second order to the
-    analytic Euler divergence: the face normals (area-weighted metric) and
-    the Jacobian enter every output."""
+
+CONV_ORDERS = {
+    "keep2": ([K2], 2), "totani": ([dict(kind=po.TERM_TOTANI)], 2), "keep4": ([K4], 4),
+    "weno_rus_lin": ([dict(kind=po.TERM_WENO, split=0, smooth=0)], 3),
+    "weno_phys_lin": ([dict(kind=po.TERM_WENO, split=1, smooth=0)], 3),
+    "fweno_lin": ([dict(kind=po.TERM_FWENO, smooth=0)], 3),
+    "first_order": ([dict(kind=po.TERM_FIRST)], 1),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CONV_ORDERS))
+def test_convective_order_on_stretched_grid(name):
+    """Every convective flux on a tanh-stretched axis converges to the Euler
+    divergence derived symbolically, at its formal order or better (WENO with
+    linear weights, enable_smooth off: third order): the face normals
+    (area-weighted metric) and the Jacobian enter every output.  The nonlinear
+    WENO weights (epsilon = 1e-16, convective.h:301-305) do not reduce to the
+    linear ones on resolvable grids, so no order is asserted for them."""
+    terms, p = CONV_ORDERS[name]
     fields, rhs_fns = _euler_exact()
     errs = []
     for n in (8, 16, 32):
         g, q, exact = _visc_mms(n, fields, rhs_fns)
-        rhs = po.flux_div(g, GAS, _flux([K2]), q)[:, 2:-2, 2:-2, 2:-2]
+        rhs = po.flux_div(g, GAS, _flux(terms), q)[:, 2:-2, 2:-2, 2:-2]
         errs.append(np.max(np.abs(rhs - exact), axis=(0, 1, 2, 3)) / np.max(np.abs(exact), axis=(0, 1, 2, 3)))
     errs = np.array(errs)
     order = np.log2(errs[:-1] / errs[1:])
-    print("keep2 stretched errors", errs[-1], "orders", order[-1])
-    assert np.all(errs[-1] < 1e-2), errs
-    assert np.all(order[-1] > 1.8), order
+    print(name, "stretched errors", errs[-1], "orders", order[-1])
+    assert np.all(order[-1] > p - 0.2), order
