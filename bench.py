@@ -169,8 +169,11 @@ def _valu_issue(workload, ms):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # 100 timed launches (about 55 ms) after 10 untimed ones: a short run
+    # still carries the clock ramp of the first launches (20 after 3 measured
+    # about 4% slower than the steady state)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--blocks", type=int, default=16, help="blocks per axis (16 -> 256^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-chosen-form", action="store_true",
