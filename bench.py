@@ -178,7 +178,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-chosen-form", action="store_true",
                     help="skip the batched / served FP32-flux measurements")
-    ap.add_argument("--fp64-steps", type=int, default=10)
+    ap.add_argument("--fp64-steps", type=int, default=30)
     ap.add_argument("--fp64-sample", type=int, default=12, help="ensemble cases timed on the FP64 path")
     ap.add_argument("--ensemble-cases", type=int, default=300)
     ap.add_argument("--ensemble-steps", type=int, default=500)

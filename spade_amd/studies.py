@@ -123,15 +123,16 @@ def run_ensemble(specs, steps, tune=True):
     return out
 
 
-def fp64_rate(spec, steps):
-    """Cell-updates/s of the plain FP64 path on one case (initial state from the engine);
-    returns (cell-updates, seconds)."""
+def fp64_rate(spec, steps, warmup=3):
+    """Cell-updates/s of the plain FP64 path on one case (initial state from the engine),
+    timed over `steps` SSP-RK3 steps after `warmup` untimed ones; returns (cell-updates,
+    seconds)."""
     b = Batch(spec.blocks * BYTES_PER_BLOCK + (16 << 20))
     cid = b.add(spec)
     cons0, dt = b.download(cid), b.info(cid)["dt"]
     b.close()
     tw = FP64Twin(spec, cons0, dt)
-    tw.it.advance(1)
+    tw.it.advance(warmup)
     tw.grid.ctx.sync()
     tw.grid.ctx.timer_start()
     tw.it.advance(steps)
