@@ -125,6 +125,69 @@ def cpu_baseline_all_cores(sample_blocks=4, reps=5, max_procs=16):
                       f"median of {reps} each; sum of their rates"}
 
 
+def rows_measure(sa, g, q, exch, gas, flux, reps=10):
+    """The path's other rows on the same state, through the library's API: the
+    stand-alone ghost exchange, one transform-aware SSP-RK3 step, the CFL and
+    kinetic-energy / enstrophy reductions, the wall fill and the binary
+    checkpoint.  Device rows are timed with HIP events on the library's
+    stream (mean of reps calls after one untimed call; a reduction's time
+    includes copying its partial results back); algorithmic bytes per call
+    over that time give GB/s.  The checkpoint crosses PCIe and host memory and
+    is timed on the host.  Leaves q changed (RK steps, wall ghosts)."""
+    nc, nbs, ng = g.cells_in_block, g.num_blocks, g.num_exch
+    nblk = nbs[0] * nbs[1] * nbs[2]
+    stored = (nc[0] + 2 * ng) * (nc[1] + 2 * ng) * (nc[2] + 2 * ng)
+    interior = nc[0] * nc[1] * nc[2]
+    cells = interior * nblk
+
+    def timed(fn, n=reps):
+        fn()
+        g.ctx.sync()
+        g.ctx.timer_start()
+        for _ in range(n):
+            fn()
+        return g.ctx.timer_stop() / n
+
+    def entry(ms, nbytes, what):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"ms": round(ms, 4), "bytes": int(nbytes), "gb_s": round(gbs, 1),
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "bytes_are": what}
+
+    out = {}
+    out["exchange"] = entry(timed(lambda: exch.exchange(q)), 80.0 * (stored - interior) * nblk,
+                            "each ghost cell's 5 doubles read from its source block and written")
+    out["max_wavespeed"] = entry(timed(lambda: sa.max_wavespeed(q, gas)), 32.0 * cells,
+                                 "T, u, v, w of every interior cell")
+    out["diagnostics"] = entry(timed(lambda: sa.diagnostics(q, gas)), 40.0 * cells,
+                               "5 doubles of every interior cell (neighbours from cache)")
+    nwall = (nbs[0] * nbs[2]) * (nc[0] + 2 * ng) * ng * (nc[2] + 2 * ng)
+    out["wall_fill"] = entry(timed(lambda: sa.boundary_fill_wall(q, 2, 1.0)), 80.0 * nwall,
+                             "each y-min ghost cell's 5 doubles read from its mirror cell and written")
+    it = sa.Integrator(q, flux, 1e-4, scheme=sa.ssprk3)
+    ms = timed(lambda: it.advance(1), n=5)
+    out["ssprk3_step"] = {"ms": round(ms, 4), "cell_updates_per_s": round(cells / (ms * 1e-3), 1),
+                          "note": "3 flux_div + 3 transform-aware stage updates writing the exchanged ghosts"}
+    import shutil
+    import tempfile
+    tmp = tempfile.mkdtemp(dir="/dev/shm" if os.path.isdir("/dev/shm") else None)
+    try:
+        path = os.path.join(tmp, "checkpoint.bin")
+        t0 = time.perf_counter()
+        sa.binary_write(path, q)
+        tw = time.perf_counter() - t0
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        sa.binary_read(path, q)
+        tr = time.perf_counter() - t0
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    out["binary_write"] = {"ms": round(tw * 1e3, 1), "gb_s": round(size / tw / 1e9, 2),
+                           "note": "host image of the array (reference layout) to a file in host memory"}
+    out["binary_read"] = {"ms": round(tr * 1e3, 1), "gb_s": round(size / tr / 1e9, 2),
+                          "note": "the same file back into the device array"}
+    return out
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -176,6 +239,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--blocks", type=int, default=16, help="blocks per axis (16 -> 256^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-rows", action="store_true",
+                    help="skip the timings of the exchange, RK step, reductions, wall fill and checkpoint")
     ap.add_argument("--no-chosen-form", action="store_true",
                     help="skip the batched / served FP32-flux measurements")
     ap.add_argument("--fp64-steps", type=int, default=30)
@@ -277,6 +342,8 @@ def main():
         g.ctx.sync()
         # the runtime copies the whole device allocation
         line["copy_check_bytes"] = 8 * g.device_size
+    if not args.no_rows:
+        line["rows"] = rows_measure(sa, g, q, exch, gas, flux)
     del q, rhs
     if not args.no_chosen_form:
         line["chosen_form"] = chosen_form(args, log=lambda m: print(m, file=sys.stderr, flush=True))
