@@ -240,13 +240,21 @@ void Batch::build_graph()
 
 void Batch::collect_diag(int first_step, int nsteps)
 {
-    // rows first_step .. first_step+nsteps-1 of the history ring
-    std::vector<double> rows((size_t)nsteps*2*max_slots);
-    for (int t = 0; t < nsteps; ++t)
+    // rows first_step .. first_step+nsteps-1 of the history ring, each cut to
+    // the slots below the highest one in use (freed slots are reused first,
+    // so that bound stays near the number of resident blocks, not the pool's)
+    size_t hi = 0;
+    for (auto& kv : cases)
+        for (int s : kv.second.slots) hi = std::max(hi, (size_t)s + 1);
+    std::vector<double> rows((size_t)nsteps*2*hi);
+    if (hi > 0)
     {
-        const int h = (first_step + t) % hist_len;
-        SPD_HIP(hipMemcpyAsync(rows.data() + (size_t)t*2*max_slots, d_diag + (size_t)h*2*max_slots,
-                               sizeof(double)*2*max_slots, hipMemcpyDeviceToHost, ctx->stream));
+        for (int t = 0; t < nsteps; ++t)
+        {
+            const int h = (first_step + t) % hist_len;
+            SPD_HIP(hipMemcpyAsync(rows.data() + (size_t)t*2*hi, d_diag + (size_t)h*2*max_slots,
+                                   sizeof(double)*2*hi, hipMemcpyDeviceToHost, ctx->stream));
+        }
     }
     SPD_HIP(hipStreamSynchronize(ctx->stream));
     for (auto& kv : cases)
@@ -258,8 +266,8 @@ void Batch::collect_diag(int first_step, int nsteps)
             double ke = 0.0, en = 0.0;
             for (int s : c.slots)   // fixed (case block) order: identical alone or in a mix
             {
-                ke += rows[(size_t)t*2*max_slots + 2*(size_t)s + 0];
-                en += rows[(size_t)t*2*max_slots + 2*(size_t)s + 1];
+                ke += rows[(size_t)t*2*hi + 2*(size_t)s + 0];
+                en += rows[(size_t)t*2*hi + 2*(size_t)s + 1];
             }
             c.ke.push_back(ke/ncell);
             c.en.push_back(en/ncell);
