@@ -6,29 +6,91 @@
 namespace spd {
 namespace {
 
-// One thread per stored cell of a block; ghost cells copy from the block that
-// owns them (26-neighbour table built on the host, grid/cartesian_blocks.h:
-// 73-105, exchange_pattern.h:382-424).  Region r = (ex+1) + 3(ey+1) + 9(ez+1)
-// where e_d = -1 below the interior, +1 above, 0 inside.
+// Ghost exchange (grid/exchange_pattern.h:382-424): every ghost cell copies
+// the cell of the neighbour block that owns it (26-neighbour table built on
+// the host, grid/cartesian_blocks.h:73-105; region (ex+1) + 3(ey+1) + 9(ez+1),
+// e_d = -1 below the interior, +1 above, 0 inside; -1 = not exchanged).  The
+// items of a block, in storage order, one per thread (grid: items / 256 x
+// blocks; every thread's copy is independent of the others', so the loads of
+// all items are in flight together):
+//   (i) the ghost rows of the core part -- rows (j, k) with j or k outside
+//       the interior: the 2 ng planes below / above in k, then the 2 ng rows
+//       below / above in j of each interior plane -- one double of a row per
+//       item, so a wave copies whole rows (nc0 contiguous doubles);
+//  (ii) the x-halo: one item per (side, row) copies the row's ng x-ghost
+//       cells on that side, whose sources are adjacent interior cells of one
+//       row of the x-neighbour (i = -1-l <- nc0-1-l, i = nc0+l <- l);
+//       consecutive items write consecutive doubles of a section.
+__host__ __device__ inline int exchange_items(const DevGeom& G)
+{
+    return (2*G.ng*G.N[1] + 2*G.ng*G.nc[2])*G.nc[0] + 2*G.N[1]*G.N[2];
+}
+
 __global__ void __launch_bounds__(256)
 k_exchange(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a)
 {
-    const int per_blk = G.N[0]*G.N[1]*G.N[2];
     const int lb = blockIdx.y;
-    const int c = blockIdx.x*blockDim.x + threadIdx.x;
-    if (c >= per_blk) return;
-    const int i = c % G.N[0] - G.ng;
-    const int j = (c / G.N[0]) % G.N[1] - G.ng;
-    const int k = c / (G.N[0]*G.N[1]) - G.ng;
-    const int ex = (i < 0) ? -1 : (i >= G.nc[0] ? 1 : 0);
-    const int ey = (j < 0) ? -1 : (j >= G.nc[1] ? 1 : 0);
-    const int ez = (k < 0) ? -1 : (k >= G.nc[2] ? 1 : 0);
-    if (ex == 0 && ey == 0 && ez == 0) return;
-    const int src = nbr[27*lb + (ex + 1) + 3*(ey + 1) + 9*(ez + 1)];
+    const int t = blockIdx.x*blockDim.x + threadIdx.x;
+    const int n0 = G.nc[0], n1 = G.nc[1], n2 = G.nc[2], ng = G.ng;
+    const int N1 = G.N[1];
+    const int rows = N1*G.N[2];
+    const long long sv = G.sv;
+    const int* nb = nbr + 27*lb;
+    double* __restrict__ blk = a + (long long)lb*G.sb;
+    auto region = [](int x, int n) { return x < 0 ? -1 : (x >= n ? 1 : 0); };
+
+    const int plane_rows = 2*ng*N1;
+    const int core_items = (plane_rows + 2*ng*n2)*n0;
+    if (t < core_items)
+    {
+        const int i = t % n0, g = t / n0;
+        int j, k;
+        if (g < plane_rows)
+        {
+            const int kk = g / N1;
+            j = g - kk*N1 - ng;
+            k = kk < ng ? kk - ng : n2 + kk - ng;
+        }
+        else
+        {
+            const int h = g - plane_rows;
+            const int jj = h % (2*ng);
+            k = h / (2*ng);
+            j = jj < ng ? jj - ng : n1 + jj - ng;
+        }
+        const int ey = region(j, n1), ez = region(k, n2);
+        const int src = nb[1 + 3*(ey + 1) + 9*(ez + 1)];
+        if (src < 0) return;
+        const int r = (k + ng)*N1 + (j + ng);
+        const int sr = (k - ez*n2 + ng)*N1 + (j - ey*n1 + ng);
+        const double* s = a + (long long)src*G.sb + (long long)sr*n0 + i;
+        double* d = blk + (long long)r*n0 + i;
+        double q[5];
+        #pragma unroll
+        for (int v = 0; v < 5; ++v) q[v] = s[v*sv];
+        #pragma unroll
+        for (int v = 0; v < 5; ++v) d[v*sv] = q[v];
+        return;
+    }
+    const int x = t - core_items;
+    if (x >= 2*rows) return;
+    const int side = x / rows, r = x - side*rows;
+    const int j = r % N1 - ng, k = r / N1 - ng;
+    const int ex = side ? 1 : -1, ey = region(j, n1), ez = region(k, n2);
+    const int src = nb[(ex + 1) + 3*(ey + 1) + 9*(ez + 1)];
     if (src < 0) return;
-    const int si = i - ex*G.nc[0], sj = j - ey*G.nc[1], sk = k - ez*G.nc[2];
-    #pragma unroll
-    for (int v = 0; v < 5; ++v) a[cell_index(G, lb, v, i, j, k)] = a[cell_index(G, src, v, si, sj, sk)];
+    const int sr = (k - ez*n2 + ng)*N1 + (j - ey*n1 + ng);
+    const double* s = a + (long long)src*G.sb + (long long)sr*n0;
+    for (int l = 0; l < ng; ++l)
+    {
+        const int si = side ? l : n0 - 1 - l;
+        double* d = blk + G.xh + (long long)(2*l + side)*rows + r;
+        double q[5];
+        #pragma unroll
+        for (int v = 0; v < 5; ++v) q[v] = s[si + v*sv];
+        #pragma unroll
+        for (int v = 0; v < 5; ++v) d[v*sv] = q[v];
+    }
 }
 
 // boundary_fill with the isothermal no-slip mirror (grid/boundary_fill.h:
@@ -184,9 +246,8 @@ void diagnostics(const Grid& g, const double* q, double R, double& ke, double& e
 
 void launch_exchange(const Grid& g, double* a, hipStream_t s)
 {
-    const DevGeom& G = g.geom;
-    const int per_blk = G.N[0]*G.N[1]*G.N[2];
-    hipLaunchKernelGGL(k_exchange, dim3((per_blk + 255)/256, G.nblk), dim3(256), 0, s, G, g.d_nbr, a);
+    const int items = exchange_items(g.geom);
+    hipLaunchKernelGGL(k_exchange, dim3((items + 255)/256, g.geom.nblk), dim3(256), 0, s, g.geom, g.d_nbr, a);
     SPD_LAUNCH_CHECK();
 }
 
