@@ -322,7 +322,7 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     const double ix = G.inv_dx[3*lb + 0], iy = G.inv_dx[3*lb + 1], iz = G.inv_dx[3*lb + 2];
     const int nz = G.nc[2];
     const long long sv = G.sv;
-    const int N1 = G.N[1], ng = G.ng;
+    const int N1 = G.N[1];
     // plane (k) strides of a core column and of an x-halo column
     const long long plane = (long long)G.nc[0]*N1, xplane = (long long)N1;
 

@@ -166,10 +166,10 @@ k_fill(double* __restrict__ a, long long n, double value)
     if (c < n) a[c] = value;
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(1024)
 k_max_wavespeed(DevGeom G, const double* __restrict__ q, double gamma, double R, double* __restrict__ part)
 {
-    __shared__ double red[256];
+    __shared__ double red[1024];
     const int per_blk = G.nc[0]*G.nc[1]*G.nc[2];
     const int lb = blockIdx.y;
     double m = 0.0;
@@ -185,7 +185,7 @@ k_max_wavespeed(DevGeom G, const double* __restrict__ q, double gamma, double R,
     }
     red[threadIdx.x] = m;
     __syncthreads();
-    for (int w = 128; w > 0; w >>= 1)
+    for (int w = 512; w > 0; w >>= 1)
     {
         if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
@@ -194,11 +194,12 @@ k_max_wavespeed(DevGeom G, const double* __restrict__ q, double gamma, double R,
 }
 
 // Per-block sums of rho |u|^2 / 2 and |omega|^2 / 2 (omega from central
-// differences; needs current ghosts).  One workgroup per block, fixed-order tree.
-__global__ void __launch_bounds__(256)
+// differences; needs current ghosts).  One workgroup of 1024 threads per block
+// (four times the loads in flight of 256), fixed-order tree.
+__global__ void __launch_bounds__(1024)
 k_diag(DevGeom G, const double* __restrict__ q, double R, double* __restrict__ part)
 {
-    __shared__ double red[2][256];
+    __shared__ double red[2][1024];
     const int lb = blockIdx.x;
     const int per_blk = G.nc[0]*G.nc[1]*G.nc[2];
     const double ix = G.inv_dx[3*lb], iy = G.inv_dx[3*lb + 1], iz = G.inv_dx[3*lb + 2];
@@ -217,7 +218,7 @@ k_diag(DevGeom G, const double* __restrict__ q, double R, double* __restrict__ p
     }
     red[0][threadIdx.x] = ke; red[1][threadIdx.x] = en;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1)
+    for (int s = 512; s > 0; s >>= 1)
     {
         if (threadIdx.x < s) { red[0][threadIdx.x] += red[0][threadIdx.x + s]; red[1][threadIdx.x] += red[1][threadIdx.x + s]; }
         __syncthreads();
@@ -232,7 +233,7 @@ void diagnostics(const Grid& g, const double* q, double R, double& ke, double& e
     const DevGeom& G = g.geom;
     double* part = nullptr;
     SPD_HIP(hipMallocAsync((void**)&part, sizeof(double)*2*G.nblk, s));
-    hipLaunchKernelGGL(k_diag, dim3(G.nblk), dim3(256), 0, s, G, q, R, part);
+    hipLaunchKernelGGL(k_diag, dim3(G.nblk), dim3(1024), 0, s, G, q, R, part);
     SPD_LAUNCH_CHECK();
     std::vector<double> h(2*(size_t)G.nblk);
     SPD_HIP(hipMemcpyAsync(h.data(), part, sizeof(double)*2*G.nblk, hipMemcpyDeviceToHost, s));
@@ -288,12 +289,11 @@ void launch_fill(const Grid& g, double* a, double value, hipStream_t s)
 double reduce_max_wavespeed(const Grid& g, const double* q, double gamma, double R, hipStream_t s)
 {
     const DevGeom& G = g.geom;
-    const int per_blk = G.nc[0]*G.nc[1]*G.nc[2];
-    const int gx = std::min(8, (per_blk + 255)/256);
+    const int gx = 1;   // one workgroup of 1024 per block: one partial per block
     const size_t np = (size_t)gx*G.nblk;
     double* part = nullptr;
     SPD_HIP(hipMallocAsync((void**)&part, np*sizeof(double), s));
-    hipLaunchKernelGGL(k_max_wavespeed, dim3(gx, G.nblk), dim3(256), 0, s, G, q, gamma, R, part);
+    hipLaunchKernelGGL(k_max_wavespeed, dim3(gx, G.nblk), dim3(1024), 0, s, G, q, gamma, R, part);
     SPD_LAUNCH_CHECK();
     std::vector<double> h(np);
     SPD_HIP(hipMemcpyAsync(h.data(), part, np*sizeof(double), hipMemcpyDeviceToHost, s));
