@@ -469,6 +469,16 @@ void launch_batch_stage(const BatchDev& B, int law, const double* Qin, const dou
     SPD_LAUNCH_CHECK();
 }
 
+int batch_stage_blocks_per_cu(int law)
+{
+    int n = 0;
+    if (law == SPD_LAW_POWER)
+        SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SPD_LAW_POWER, true>, 256, 0));
+    else
+        SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SPD_LAW_CONST, true>, 256, 0));
+    return std::max(1, n);
+}
+
 void launch_batch_step_count(const BatchDev& B, hipStream_t s)
 {
     hipLaunchKernelGGL(k_step_count, dim3(1), dim3(64), 0, s, B.step);

@@ -42,6 +42,9 @@ struct BatchDev
 void launch_batch_stage(const BatchDev& B, int law, const double* Qin, const double* Q0, double* Qout,
                         double alpha, double beta, int want_diag, int grid, hipStream_t s);
 void launch_batch_step_count(const BatchDev& B, hipStream_t s);
+// Workgroups of the stage kernel (for law) resident per CU at once: the
+// default persistent grid is this many per CU
+int batch_stage_blocks_per_cu(int law);
 // d_stage: per block {slot, case table, 27 neighbour slots}; also fills B.nbr and B.slot_case
 void launch_batch_init(const BatchDev& B, const int* d_stage, int nslots, const CaseParams& P, const CaseIC& ic,
                        double* Q, hipStream_t s);

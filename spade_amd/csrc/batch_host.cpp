@@ -50,7 +50,8 @@ Batch::Batch(Ctx* c, long long budget_bytes, int law_in, int hist) : ctx(c), law
     int cus = 0;
     SPD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     num_cu = std::max(1, cus);
-    grid = 2*num_cu;
+    // as many persistent workgroups as fit at once (spd_batch_tune may pick another)
+    grid = batch_stage_blocks_per_cu(law)*num_cu;
     dev = BatchDev{d_worklist, d_small + 0, d_slot_case, d_nbr, d_cases, d_diag, d_small + 1, max_slots, hist_len};
 }
 
