@@ -45,8 +45,21 @@ void validate_flux(const spd_flux_desc* f, int ng)
                 break;
             case SPD_TERM_WENO: case SPD_TERM_FWENO:
                 require(ng >= 2, "WENO stencils need 2 ghost layers");
+                require(t.kind == SPD_TERM_FWENO || t.split == SPD_SPLIT_RUSANOV || t.split == SPD_SPLIT_PHYS,
+                        "WENO split must be SPD_SPLIT_RUSANOV or SPD_SPLIT_PHYS");
                 break;
-            case SPD_TERM_TOTANI: case SPD_TERM_PDISS: case SPD_TERM_FIRST: case SPD_TERM_VISC:
+            case SPD_TERM_PDISS:
+                require(ng >= 1, "flux needs at least one ghost layer");
+                require(t.sensor == SPD_SENSOR_CONST || t.sensor == SPD_SENSOR_DUCROS,
+                        "pressure_diss sensor must be SPD_SENSOR_CONST or SPD_SENSOR_DUCROS");
+                break;
+            case SPD_TERM_VISC:
+                require(ng >= 1, "flux needs at least one ghost layer");
+                require(t.law == SPD_LAW_CONST || t.law == SPD_LAW_POWER, "viscous law must be SPD_LAW_CONST or SPD_LAW_POWER");
+                require(t.prandtl > 0.0, "viscous law needs a Prandtl number > 0");
+                require(t.law == SPD_LAW_CONST || t.T_ref > 0.0, "power law needs T_ref > 0");
+                break;
+            case SPD_TERM_TOTANI: case SPD_TERM_FIRST:
                 require(ng >= 1, "flux needs at least one ghost layer");
                 break;
             default:
