@@ -99,10 +99,13 @@ k_exchange(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a)
 __global__ void __launch_bounds__(256)
 k_wall_fill(DevGeom G, double* __restrict__ a, int idir, int pm, double t_wall)
 {
-    const int lb = blockIdx.y;
-    int b[3] = {lb % G.nb[0], (lb / G.nb[0]) % G.nb[1], lb / (G.nb[0]*G.nb[1])};
-    if (pm == 0 && b[idir] != 0) return;
-    if (pm == 1 && b[idir] != G.nb[idir] - 1) return;
+    // blockIdx.y enumerates the blocks on the face: (b[d1], b[d2]), d1 < d2 the other axes
+    const int d1 = idir == 0 ? 1 : 0, d2 = idir == 2 ? 1 : 2;
+    int b[3];
+    b[idir] = pm == 0 ? 0 : G.nb[idir] - 1;
+    b[d1] = blockIdx.y % G.nb[d1];
+    b[d2] = blockIdx.y / G.nb[d1];
+    const int lb = b[0] + G.nb[0]*(b[1] + G.nb[1]*b[2]);
     int ext[3] = {G.N[0], G.N[1], G.N[2]};
     ext[idir] = G.ng;
     const int c = blockIdx.x*blockDim.x + threadIdx.x;
@@ -259,7 +262,9 @@ void launch_wall_fill(const Grid& g, double* a, int face, double t_wall, hipStre
     int ext[3] = {G.N[0], G.N[1], G.N[2]};
     ext[idir] = G.ng;
     const int n = ext[0]*ext[1]*ext[2];
-    hipLaunchKernelGGL(k_wall_fill, dim3((n + 255)/256, G.nblk), dim3(256), 0, s, G, a, idir, pm, t_wall);
+    const int d1 = idir == 0 ? 1 : 0, d2 = idir == 2 ? 1 : 2;
+    const int face_blocks = G.nb[d1]*G.nb[d2];
+    hipLaunchKernelGGL(k_wall_fill, dim3((n + 255)/256, face_blocks), dim3(256), 0, s, G, a, idir, pm, t_wall);
     SPD_LAUNCH_CHECK();
 }
 
