@@ -276,6 +276,8 @@ int spd_grid_destroy(spd_grid_t grid)
         SPD_HIP(hipFree(grid->g.d_push));
         SPD_HIP(hipFree(grid->g.d_start));
         if (grid->g.d_stage) SPD_HIP(hipFree(grid->g.d_stage));
+        if (grid->g.d_red) SPD_HIP(hipFree(grid->g.d_red));
+        if (grid->g.h_red) SPD_HIP(hipHostFree(grid->g.h_red));
         delete grid;
     });
 }

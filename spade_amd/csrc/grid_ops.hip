@@ -229,19 +229,44 @@ k_diag(DevGeom G, const double* __restrict__ q, double R, double* __restrict__ p
     if (threadIdx.x == 0) { part[2*lb] = red[0][0]; part[2*lb + 1] = red[1][0]; }
 }
 
+// max of n partials by one workgroup (order-free: max is exact)
+__global__ void __launch_bounds__(1024)
+k_max_final(const double* __restrict__ part, int n, double* __restrict__ out)
+{
+    __shared__ double red[1024];
+    double m = 0.0;
+    for (int t = threadIdx.x; t < n; t += blockDim.x) m = fmax(m, part[t]);
+    red[threadIdx.x] = m;
+    __syncthreads();
+    for (int w = 512; w > 0; w >>= 1)
+    {
+        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// Per-grid reduction buffers, allocated on first use and kept with the grid:
+// partials on the device (2 nblk + 1 doubles) and a pinned host copy, so a
+// reduction allocates nothing and copies back through pinned memory.
+void ensure_red(const Grid& g)
+{
+    Grid& m = const_cast<Grid&>(g);
+    if (!m.d_red) SPD_HIP(hipMalloc((void**)&m.d_red, sizeof(double)*(2*(size_t)g.geom.nblk + 1)));
+    if (!m.h_red) SPD_HIP(hipHostMalloc((void**)&m.h_red, sizeof(double)*(2*(size_t)g.geom.nblk + 1)));
+}
+
 } // namespace
 
 void diagnostics(const Grid& g, const double* q, double R, double& ke, double& en, hipStream_t s)
 {
     const DevGeom& G = g.geom;
-    double* part = nullptr;
-    SPD_HIP(hipMallocAsync((void**)&part, sizeof(double)*2*G.nblk, s));
-    hipLaunchKernelGGL(k_diag, dim3(G.nblk), dim3(1024), 0, s, G, q, R, part);
+    ensure_red(g);
+    hipLaunchKernelGGL(k_diag, dim3(G.nblk), dim3(1024), 0, s, G, q, R, g.d_red);
     SPD_LAUNCH_CHECK();
-    std::vector<double> h(2*(size_t)G.nblk);
-    SPD_HIP(hipMemcpyAsync(h.data(), part, sizeof(double)*2*G.nblk, hipMemcpyDeviceToHost, s));
-    SPD_HIP(hipFreeAsync(part, s));
+    SPD_HIP(hipMemcpyAsync(g.h_red, g.d_red, sizeof(double)*2*G.nblk, hipMemcpyDeviceToHost, s));
     SPD_HIP(hipStreamSynchronize(s));
+    const double* h = g.h_red;
     double a = 0.0, b = 0.0;
     for (int lb = 0; lb < G.nblk; ++lb) { a += h[2*lb]; b += h[2*lb + 1]; }
     const double n = (double)G.nc[0]*G.nc[1]*G.nc[2]*G.nblk;
@@ -294,19 +319,15 @@ void launch_fill(const Grid& g, double* a, double value, hipStream_t s)
 double reduce_max_wavespeed(const Grid& g, const double* q, double gamma, double R, hipStream_t s)
 {
     const DevGeom& G = g.geom;
-    const int gx = 1;   // one workgroup of 1024 per block: one partial per block
-    const size_t np = (size_t)gx*G.nblk;
-    double* part = nullptr;
-    SPD_HIP(hipMallocAsync((void**)&part, np*sizeof(double), s));
-    hipLaunchKernelGGL(k_max_wavespeed, dim3(gx, G.nblk), dim3(1024), 0, s, G, q, gamma, R, part);
+    ensure_red(g);
+    // one workgroup of 1024 per block: one partial per block, then their max on the device
+    hipLaunchKernelGGL(k_max_wavespeed, dim3(1, G.nblk), dim3(1024), 0, s, G, q, gamma, R, g.d_red);
     SPD_LAUNCH_CHECK();
-    std::vector<double> h(np);
-    SPD_HIP(hipMemcpyAsync(h.data(), part, np*sizeof(double), hipMemcpyDeviceToHost, s));
-    SPD_HIP(hipFreeAsync(part, s));
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(1024), 0, s, g.d_red, G.nblk, g.d_red + 2*(size_t)G.nblk);
+    SPD_LAUNCH_CHECK();
+    SPD_HIP(hipMemcpyAsync(g.h_red, g.d_red + 2*(size_t)G.nblk, sizeof(double), hipMemcpyDeviceToHost, s));
     SPD_HIP(hipStreamSynchronize(s));
-    double m = 0.0;
-    for (double x : h) m = std::max(m, x);
-    return m;
+    return g.h_red[0];
 }
 
 } // namespace spd

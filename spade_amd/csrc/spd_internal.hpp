@@ -63,6 +63,8 @@ struct Grid
     int* d_start = nullptr;       // [lb] first plane the LDS-staged flux kernel left to the fix-up kernel
     double* d_stage = nullptr;    // host-layout staging for upload / download (stage_blocks blocks), lazily allocated
     int stage_blocks = 0;
+    double* d_red = nullptr;      // reduction partials, 2 nblk + 1 doubles, lazily allocated
+    double* h_red = nullptr;      // pinned host copy of them
     std::vector<double> h_dx, h_inv_dx, h_bmin;
 };
 

@@ -69,3 +69,27 @@ def test_batched_diagnostics_tgv_closed_form():
     print("batched diagnostics", ke[0], ke_ref, en[0], en_ref, flush=True)
     assert abs(ke[0] - ke_ref) <= 1e-13 * ke_ref
     assert abs(en[0] - en_ref) <= 1e-13 * en_ref
+
+
+def test_max_wavespeed_equals_host_value():
+    """max(|u| + sqrt(gamma R T)) over the interior cells (the CFL wave speed of
+    development/cuda-tgv/main.cc:155-172): the same expression in the same
+    order, correctly rounded square roots and an exact max, so the device value
+    equals the host one bit for bit; twice, so the kept buffers are reused."""
+    import spade_amd as sa
+    sa.load()
+    nb, nc, ng = (3, 2, 2), (16, 8, 12), 2
+    g = sa.CartesianGrid(nc, nb, ((0.0, L),) * 3, num_exch=ng)
+    rng = np.random.default_rng(11)
+    gas = sa.IdealGas(1.4, R)
+    for trial in range(2):
+        host = np.zeros(g.host_shape)
+        host[..., 0] = R * (1.0 + 0.1 * rng.random(host.shape[:-1]))
+        host[..., 1] = 1.0 + 0.5 * rng.random(host.shape[:-1])
+        host[..., 2:] = rng.standard_normal(host.shape[:-1] + (3,))
+        q = sa.GridArray(g).upload(host)
+        inner = host[:, ng:-ng, ng:-ng, ng:-ng]
+        T, u, v, w = inner[..., 1], inner[..., 2], inner[..., 3], inner[..., 4]
+        want = np.max(np.sqrt(1.4 * R * T) + np.sqrt(u * u + v * v + w * w))
+        got = sa.max_wavespeed(q, gas)
+        assert got == want, (trial, got, want)
