@@ -204,19 +204,30 @@ k_diag(DevGeom G, const double* __restrict__ q, double R, double* __restrict__ p
 {
     __shared__ double red[2][1024];
     const int lb = blockIdx.x;
-    const int per_blk = G.nc[0]*G.nc[1]*G.nc[2];
+    const int n0 = G.nc[0], n1 = G.nc[1], ng = G.ng, N1 = G.N[1];
+    const int per_blk = n0*n1*G.nc[2];
+    const long long sv = G.sv, rows = (long long)N1*G.N[2];
+    const double* __restrict__ B = q + (long long)lb*G.sb;
     const double ix = G.inv_dx[3*lb], iy = G.inv_dx[3*lb + 1], iz = G.inv_dx[3*lb + 2];
+    // neighbours in j and k are core cells (ghost rows included) at -+n0 and
+    // -+N1 n0; in i, core cells inside the block, else the first x-halo
+    // section of that side (i = -1: section 0, i = n0: section 1)
+    const long long dj = n0, dk = (long long)N1*n0;
     double ke = 0.0, en = 0.0;
     for (int c = threadIdx.x; c < per_blk; c += blockDim.x)
     {
-        const int i = c % G.nc[0], j = (c / G.nc[0]) % G.nc[1], k = c / (G.nc[0]*G.nc[1]);
-        auto Q = [&](int v, int a, int b, int d) { return q[cell_index(G, lb, v, a, b, d)]; };
-        const double p = Q(0, i, j, k), T = Q(1, i, j, k);
-        const double u = Q(2, i, j, k), v = Q(3, i, j, k), w = Q(4, i, j, k);
+        const int i = c % n0, j = (c / n0) % n1, k = c / (n0*n1);
+        const long long row = (long long)(k + ng)*N1 + (j + ng);
+        const long long o = row*n0 + i;
+        const long long om = i > 0 ? o - 1 : G.xh + row;
+        const long long op = i + 1 < n0 ? o + 1 : G.xh + rows + row;
+        auto Q = [&](int v, long long off) { return B[v*sv + off]; };
+        const double p = Q(0, o), T = Q(1, o);
+        const double u = Q(2, o), v = Q(3, o), w = Q(4, o);
         ke += 0.5*(p/(R*T))*(u*u + v*v + w*w);
-        const double ox = 0.5*((Q(4, i, j + 1, k) - Q(4, i, j - 1, k))*iy - (Q(3, i, j, k + 1) - Q(3, i, j, k - 1))*iz);
-        const double oy = 0.5*((Q(2, i, j, k + 1) - Q(2, i, j, k - 1))*iz - (Q(4, i + 1, j, k) - Q(4, i - 1, j, k))*ix);
-        const double oz = 0.5*((Q(3, i + 1, j, k) - Q(3, i - 1, j, k))*ix - (Q(2, i, j + 1, k) - Q(2, i, j - 1, k))*iy);
+        const double ox = 0.5*((Q(4, o + dj) - Q(4, o - dj))*iy - (Q(3, o + dk) - Q(3, o - dk))*iz);
+        const double oy = 0.5*((Q(2, o + dk) - Q(2, o - dk))*iz - (Q(4, op) - Q(4, om))*ix);
+        const double oz = 0.5*((Q(3, op) - Q(3, om))*ix - (Q(2, o + dj) - Q(2, o - dj))*iy);
         en += 0.5*(ox*ox + oy*oy + oz*oz);
     }
     red[0][threadIdx.x] = ke; red[1][threadIdx.x] = en;
