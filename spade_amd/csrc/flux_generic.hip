@@ -565,14 +565,18 @@ __global__ void __launch_bounds__(256)
 k_flux_fixup(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, spd_flux_desc F, int increment,
              const int* __restrict__ start)
 {
+    const int nz = G.nc[2];
+    // a grid of at most 1024 workgroups walks the blocks.  The usual case is
+    // that none of this workgroup's blocks needs completing: every thread reads
+    // the same flags, so the whole workgroup leaves before staging anything
+    bool any = false;
+    for (int lb = blockIdx.x; lb < G.nblk; lb += gridDim.x) any |= start[lb] < nz;
+    if (!any) return;
     __shared__ DevGeom sG;                // see k_flux_div_generic
     __shared__ spd_flux_desc sF;
     if (threadIdx.x == 0) { sG = G; sF = F; }
     __syncthreads();
-    const int nz = G.nc[2];
     const int per_plane = G.nc[0]*G.nc[1];
-    // a grid of at most 1024 workgroups walks the blocks: when no block needs
-    // completing (the usual case) the launch costs a few microseconds
     for (int lb = blockIdx.x; lb < G.nblk; lb += gridDim.x)
     {
         int s = start[lb];
