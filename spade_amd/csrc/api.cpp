@@ -149,6 +149,11 @@ int spd_ctx_destroy(spd_ctx_t ctx)
     {
         if (!ctx) return;
         SPD_HIP(hipStreamSynchronize(ctx->c.stream));
+        for (int b = 0; b < 2; ++b)
+        {
+            if (ctx->c.h_io[b]) SPD_HIP(hipHostFree(ctx->c.h_io[b]));
+            if (ctx->c.io_ev[b]) SPD_HIP(hipEventDestroy(ctx->c.io_ev[b]));
+        }
         SPD_HIP(hipEventDestroy(ctx->c.ev0));
         SPD_HIP(hipEventDestroy(ctx->c.ev1));
         SPD_HIP(hipStreamDestroy(ctx->c.stream));
@@ -650,7 +655,7 @@ double map_coord(const spd_map1d& m, double xi)
     return c + h*std::tanh(m.beta*((xi - c)/h))/std::tanh(m.beta);
 }
 
-void upload_blocks(spd_array_s* arr, int lb0, int nb, const double* host)
+void upload_blocks(spd_array_s* arr, int lb0, int nb, const double* host, bool sync)
 {
     Grid& g = *arr->a.grid;
     require(lb0 >= 0 && nb >= 0 && lb0 + nb <= g.geom.nblk, "block range outside the grid");
@@ -663,10 +668,10 @@ void upload_blocks(spd_array_s* arr, int lb0, int nb, const double* host)
                                g.ctx->stream));
         launch_aos_to_soa(g, stage, arr->a.d, b0, n, g.ctx->stream);
     }
-    SPD_HIP(hipStreamSynchronize(g.ctx->stream));
+    if (sync) SPD_HIP(hipStreamSynchronize(g.ctx->stream));
 }
 
-void download_blocks(const spd_array_s* arr, int lb0, int nb, double* host)
+void download_blocks(const spd_array_s* arr, int lb0, int nb, double* host, bool sync)
 {
     Grid& g = *arr->a.grid;
     require(lb0 >= 0 && nb >= 0 && lb0 + nb <= g.geom.nblk, "block range outside the grid");
@@ -679,7 +684,7 @@ void download_blocks(const spd_array_s* arr, int lb0, int nb, double* host)
         SPD_HIP(hipMemcpyAsync(host + 5*cells*(b0 - lb0), stage, sizeof(double)*5*cells*n, hipMemcpyDeviceToHost,
                                g.ctx->stream));
     }
-    SPD_HIP(hipStreamSynchronize(g.ctx->stream));
+    if (sync) SPD_HIP(hipStreamSynchronize(g.ctx->stream));
 }
 
 } // namespace spd

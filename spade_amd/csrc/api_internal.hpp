@@ -60,7 +60,9 @@ double map_coord(const spd_map1d& m, double xi);
 // Blocks [lb0, lb0 + nb) of an array from / to host memory in the reference
 // layout (AoS, variable fastest, ghosts stored), through the grid's staging
 // buffer; both return when the host memory may be reused.
-void upload_blocks(spd_array_s* arr, int lb0, int nb, const double* host);
-void download_blocks(const spd_array_s* arr, int lb0, int nb, double* host);
+// sync = false leaves the copies in flight on the context's stream (the host
+// buffer must then stay untouched until the stream has passed them)
+void upload_blocks(spd_array_s* arr, int lb0, int nb, const double* host, bool sync = true);
+void download_blocks(const spd_array_s* arr, int lb0, int nb, double* host, bool sync = true);
 
 } // namespace spd

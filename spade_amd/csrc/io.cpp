@@ -191,6 +191,36 @@ size_t image_doubles(const Grid& g)
 
 } // namespace
 
+namespace spd {
+namespace {
+
+// The context's two pinned checkpoint buffers, each at least n doubles.
+void ensure_io_buffers(Ctx& c, size_t n)
+{
+    if (c.h_io_doubles < n)
+    {
+        for (int b = 0; b < 2; ++b)
+        {
+            if (c.h_io[b]) { SPD_HIP(hipHostFree(c.h_io[b])); c.h_io[b] = nullptr; }
+            SPD_HIP(hipHostMalloc((void**)&c.h_io[b], sizeof(double)*n));
+        }
+        c.h_io_doubles = n;
+    }
+    for (int b = 0; b < 2; ++b)
+        if (!c.io_ev[b]) SPD_HIP(hipEventCreateWithFlags(&c.io_ev[b], hipEventDisableTiming));
+}
+
+// Waits for the context's stream on every way out of a checkpoint call, so
+// no copy into or out of a pinned buffer is still running when it returns.
+struct StreamDrain
+{
+    hipStream_t s;
+    ~StreamDrain() { (void)hipStreamSynchronize(s); }
+};
+
+} // namespace
+} // namespace spd
+
 extern "C" {
 
 int spd_io_output_vtk(spd_array_t arr, const char* out_dir, const char* basename, const char* const* names)
@@ -234,16 +264,29 @@ int spd_io_binary_write(spd_array_t arr, const char* filename)
     {
         require(arr && filename, "null argument");
         Grid& g = *arr->a.grid;
+        Ctx& cx = *g.ctx;
         const size_t blk = 5*(size_t)g.geom.N[0]*g.geom.N[1]*g.geom.N[2];
         const int chunk = std::max(1, std::min(g.geom.nblk, (int)((64u << 20)/(8*blk))));
+        const int nchunks = (g.geom.nblk + chunk - 1)/chunk;
         std::ofstream f(filename, std::ios::binary | std::ios::trunc);
         if (!f) throw IoError{std::string("cannot open ") + filename + " for writing"};
-        std::vector<double> buf(blk*chunk);
-        for (int lb0 = 0; lb0 < g.geom.nblk; lb0 += chunk)
+        ensure_io_buffers(cx, blk*chunk);
+        StreamDrain drain{cx.stream};
+        // chunk c goes through pinned buffer c % 2: the device converts and
+        // copies chunk c+1 while the host writes chunk c
+        auto enqueue = [&](int c)
         {
-            const int nb = std::min(chunk, g.geom.nblk - lb0);
-            download_blocks(arr, lb0, nb, buf.data());
-            f.write(reinterpret_cast<const char*>(buf.data()), (std::streamsize)(sizeof(double)*blk*nb));
+            const int lb0 = c*chunk, nb = std::min(chunk, g.geom.nblk - lb0);
+            download_blocks(arr, lb0, nb, cx.h_io[c & 1], false);
+            SPD_HIP(hipEventRecord(cx.io_ev[c & 1], cx.stream));
+        };
+        enqueue(0);
+        for (int c = 0; c < nchunks; ++c)
+        {
+            if (c + 1 < nchunks) enqueue(c + 1);
+            SPD_HIP(hipEventSynchronize(cx.io_ev[c & 1]));
+            const int nb = std::min(chunk, g.geom.nblk - c*chunk);
+            f.write(reinterpret_cast<const char*>(cx.h_io[c & 1]), (std::streamsize)(sizeof(double)*blk*nb));
             if (!f) throw IoError{std::string("cannot write ") + filename};
         }
         f.close();
@@ -266,13 +309,19 @@ int spd_io_binary_read(spd_array_t arr, const char* filename)
                           std::to_string(sizeof(double)*image_doubles(g))};
         f.seekg(0);
         const int chunk = std::max(1, std::min(g.geom.nblk, (int)((64u << 20)/(8*blk))));
-        std::vector<double> buf(blk*chunk);
-        for (int lb0 = 0; lb0 < g.geom.nblk; lb0 += chunk)
+        Ctx& cx = *g.ctx;
+        ensure_io_buffers(cx, blk*chunk);
+        StreamDrain drain{cx.stream};
+        // chunk c goes through pinned buffer c % 2: the host reads chunk c
+        // while the device copies and converts chunk c-1
+        for (int lb0 = 0, c = 0; lb0 < g.geom.nblk; lb0 += chunk, ++c)
         {
             const int nb = std::min(chunk, g.geom.nblk - lb0);
-            f.read(reinterpret_cast<char*>(buf.data()), (std::streamsize)(sizeof(double)*blk*nb));
+            if (c >= 2) SPD_HIP(hipEventSynchronize(cx.io_ev[c & 1]));     // chunk c-2 has left this buffer
+            f.read(reinterpret_cast<char*>(cx.h_io[c & 1]), (std::streamsize)(sizeof(double)*blk*nb));
             if (!f) throw IoError{std::string("cannot read ") + filename};
-            upload_blocks(arr, lb0, nb, buf.data());
+            upload_blocks(arr, lb0, nb, cx.h_io[c & 1], false);
+            SPD_HIP(hipEventRecord(cx.io_ev[c & 1], cx.stream));
         }
     });
 }

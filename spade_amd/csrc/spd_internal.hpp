@@ -50,6 +50,11 @@ struct Ctx
     int device = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    // two pinned host buffers for the binary checkpoint, double-buffered
+    // against the device conversion and copies; allocated on first use
+    double* h_io[2] = {nullptr, nullptr};
+    size_t h_io_doubles = 0;
+    hipEvent_t io_ev[2] = {nullptr, nullptr};
 };
 
 struct Grid
