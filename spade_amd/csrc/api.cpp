@@ -504,7 +504,7 @@ int spd_max_wavespeed(spd_array_t prims, double gamma, double R, double* out)
     return guarded([&]
     {
         require(prims && out, "null argument");
-        const Grid& g = *prims->a.grid;
+        Grid& g = *prims->a.grid;
         *out = reduce_max_wavespeed(g, prims->a.d, gamma, R, g.ctx->stream);
     });
 }
@@ -515,7 +515,7 @@ int spd_diagnostics(spd_array_t prims, double R, double* ke, double* en)
     {
         require(prims && ke && en, "null argument");
         require(R > 0.0, "R must be positive");
-        const Grid& g = *prims->a.grid;
+        Grid& g = *prims->a.grid;
         diagnostics(g, prims->a.d, R, *ke, *en, g.ctx->stream);
     });
 }

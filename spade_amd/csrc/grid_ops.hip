@@ -260,16 +260,15 @@ k_max_final(const double* __restrict__ part, int n, double* __restrict__ out)
 // Per-grid reduction buffers, allocated on first use and kept with the grid:
 // partials on the device (2 nblk + 1 doubles) and a pinned host copy, so a
 // reduction allocates nothing and copies back through pinned memory.
-void ensure_red(const Grid& g)
+void ensure_red(Grid& g)
 {
-    Grid& m = const_cast<Grid&>(g);
-    if (!m.d_red) SPD_HIP(hipMalloc((void**)&m.d_red, sizeof(double)*(2*(size_t)g.geom.nblk + 1)));
-    if (!m.h_red) SPD_HIP(hipHostMalloc((void**)&m.h_red, sizeof(double)*(2*(size_t)g.geom.nblk + 1)));
+    if (!g.d_red) SPD_HIP(hipMalloc((void**)&g.d_red, sizeof(double)*(2*(size_t)g.geom.nblk + 1)));
+    if (!g.h_red) SPD_HIP(hipHostMalloc((void**)&g.h_red, sizeof(double)*(2*(size_t)g.geom.nblk + 1)));
 }
 
 } // namespace
 
-void diagnostics(const Grid& g, const double* q, double R, double& ke, double& en, hipStream_t s)
+void diagnostics(Grid& g, const double* q, double R, double& ke, double& en, hipStream_t s)
 {
     const DevGeom& G = g.geom;
     ensure_red(g);
@@ -327,7 +326,7 @@ void launch_fill(const Grid& g, double* a, double value, hipStream_t s)
     SPD_LAUNCH_CHECK();
 }
 
-double reduce_max_wavespeed(const Grid& g, const double* q, double gamma, double R, hipStream_t s)
+double reduce_max_wavespeed(Grid& g, const double* q, double gamma, double R, hipStream_t s)
 {
     const DevGeom& G = g.geom;
     ensure_red(g);

@@ -121,8 +121,9 @@ void launch_rk_transform_stage(const Grid& g, double* q, const double* const* r,
 void launch_rk_opt_s0(const Grid& g, double* q, const double* r0, double dt, double gamma, double R, hipStream_t s);
 void launch_rk_opt_s1(const Grid& g, double* q, double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s);
 void launch_rk_opt_s2(const Grid& g, double* q, const double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s);
-void diagnostics(const Grid& g, const double* q, double R, double& ke, double& en, hipStream_t s);
-double reduce_max_wavespeed(const Grid& g, const double* q, double gamma, double R, hipStream_t s);
+// both keep their reduction buffers with the grid (allocated on first use)
+void diagnostics(Grid& g, const double* q, double R, double& ke, double& en, hipStream_t s);
+double reduce_max_wavespeed(Grid& g, const double* q, double gamma, double R, hipStream_t s);
 
 // Device layout of one variable of one block ("core + x-halo"):
 //   core:   every stored row (j, k in [-ng, n+ng)), ghost rows included, holds
