@@ -188,6 +188,40 @@ def rows_measure(sa, g, q, exch, gas, flux, reps=10):
     return out
 
 
+def channel_measure(sa, reps=20):
+    """flux_div on the stretched wall-bounded channel (192 x 128 x 96 cells,
+    12 x 8 x 6 blocks of 16^3, tanh wall-normal map, cent_keep<2> +
+    pressure_diss_lr + visc_lr power law): the curvilinear path, with the
+    kernel the library selects for it, timed like the headline after the
+    exchange and both wall fills it needs.  Bytes per cell-RHS as for the
+    headline (the prims read once, the residual written once)."""
+    from spade_amd import studies
+    c = studies.channel()
+    g = sa.CartesianGrid(c["nc"], c["nb"], tuple(zip(c["lo"], c["hi"])), num_exch=c["ng"],
+                         periodic=(True, False, True), maps=(None, sa.tanh_map(*c["tanh_y"]), None))
+    gas = sa.IdealGas(c["gamma"], c["R"])
+    a, ep, et = c["pdiss"]
+    flux = sa.compose(sa.cent_keep(2, gas), sa.pressure_diss_lr(gas, sa.const_sensor(a), ep, et),
+                      sa.visc_lr(sa.power_law(c["mu"], 1.0, c["power"], c["prandtl"]), gas))
+    q = sa.GridArray(g).upload(c["host"])
+    rhs = sa.GridArray(g, fill=0.0)
+    sa.create_exchange(q).exchange(q)
+    sa.boundary_fill_wall(q, 2, c["T_wall"])
+    sa.boundary_fill_wall(q, 3, c["T_wall"])
+    for _ in range(5):
+        sa.flux_div(q, rhs, flux, increment=False)
+    g.ctx.sync()
+    g.ctx.timer_start()
+    for _ in range(reps):
+        sa.flux_div(q, rhs, flux, increment=False)
+    ms = g.ctx.timer_stop() / reps
+    cells = g.num_cells
+    gbs = BYTES_PER_CELL_RHS * cells / (ms * 1e-3) / 1e9
+    return {"ms": round(ms, 4), "mcell_rhs_per_s": round(cells / (ms * 1e-3) * 1e-6, 1), "cells": cells,
+            "kernel": sa.flux_div_kernel(q, flux), "gb_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "note": "192x128x96 stretched channel, cent_keep<2> + pressure_diss_lr + visc_lr power law, FP64"}
+
+
 def _cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -240,7 +274,8 @@ def main():
     ap.add_argument("--blocks", type=int, default=16, help="blocks per axis (16 -> 256^3)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-rows", action="store_true",
-                    help="skip the timings of the exchange, RK step, reductions, wall fill and checkpoint")
+                    help="skip the timings of the exchange, RK step, reductions, wall fill, checkpoint and "
+                         "the stretched-channel flux_div")
     ap.add_argument("--no-chosen-form", action="store_true",
                     help="skip the batched / served FP32-flux measurements")
     ap.add_argument("--fp64-steps", type=int, default=30)
@@ -258,7 +293,14 @@ def main():
                     help="instead of the benchmark: FP32-flux vs FP64 TGV run (errors every 100 steps)")
     ap.add_argument("--study-cells", type=int, default=256)
     ap.add_argument("--study-steps", type=int, default=1000)
+    ap.add_argument("--channel", action="store_true",
+                    help="instead of the benchmark: time flux_div on the 192x128x96 stretched channel only")
     args = ap.parse_args()
+    if args.channel:
+        import spade_amd as sa
+        sa.load()
+        print(json.dumps(channel_measure(sa)), flush=True)
+        return
     if args.precision_study:
         import spade_amd as sa
         from spade_amd import studies
@@ -345,6 +387,8 @@ def main():
     if not args.no_rows:
         line["rows"] = rows_measure(sa, g, q, exch, gas, flux)
     del q, rhs
+    if not args.no_rows:
+        line["rows"]["channel_flux_div"] = channel_measure(sa)
     if not args.no_chosen_form:
         line["chosen_form"] = chosen_form(args, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu_baseline:

@@ -127,8 +127,10 @@ int spd_flux_div(spd_array_t prims, spd_array_t rhs, const spd_flux_desc* flux, 
 
 /* Which kernel spd_flux_div would run for this array and flux: *kind = 1 for
  * the LDS-staged face-once kernel (cent_keep<2> [+pressure_diss_lr]
- * [+visc_lr], identity coordinates, 16x16 block cross-section), 0 for the
- * generic one-thread-per-cell kernel.  No counterpart in the reference (it has
+ * [+visc_lr], identity coordinates, 16x16 block cross-section), 2 for the
+ * one-thread-per-cell kernel specialised for such a composition on any other
+ * grid (mapped coordinates included), 0 for the generic one-thread-per-cell
+ * kernel.  No counterpart in the reference (it has
  * a single algorithm per trait); for tests and benchmarks. */
 int spd_flux_div_kernel(spd_array_t prims, const spd_flux_desc* flux, int* kind);
 

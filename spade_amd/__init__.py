@@ -321,11 +321,13 @@ def flux_div(prims, rhs, flux, increment=True):
 
 
 def flux_div_kernel(prims, flux):
-    """'fast' (LDS face-once kernel) or 'generic' for this array and flux."""
+    """'fast' (LDS face-once kernel), 'specialized' (per-cell kernel compiled for
+    a composition of cent_keep<2>, pressure_diss_lr and visc_lr) or 'generic'
+    (per-cell kernel for every functor) for this array and flux."""
     d = flux_descriptor(flux)
     kind = C.c_int()
     check(load().spd_flux_div_kernel(prims._h, C.byref(d), C.byref(kind)))
-    return "fast" if kind.value == 1 else "generic"
+    return {1: "fast", 2: "specialized"}.get(kind.value, "generic")
 
 
 class ExchangeHandle:

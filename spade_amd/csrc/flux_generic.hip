@@ -591,14 +591,126 @@ k_flux_fixup(DevGeom G, const double* __restrict__ prims, double* __restrict__ r
     }
 }
 
+// The per-cell kernel specialised for a composition of cent_keep<2>,
+// pressure_diss_lr and visc_lr (a subsequence of the three, in that order) on
+// any geometry, in particular mapped coordinates, which the LDS-staged kernel
+// does not take.  The terms are known at compile time and the kernel is
+// flattened: every term and helper is inlined, so the small arrays the term
+// functions fill (faces, gradients, metrics) stay in registers instead of the
+// per-lane stack that the run-time dispatch of k_flux_div_generic needs.  The
+// arithmetic is the same code in the same order as generic_cell.
+template <int KEEP, int PD, int VS>
+__device__ inline void spec_face(const Cell& C, const FaceIdx& f, const spd_flux_desc& F, double acc[5])
+{
+    // F0 + (F1 + F2): the last term first, then each earlier one plus the sum
+    if constexpr (VS) f_visc(C, f, F.terms[KEEP + PD], F.R, F.gamma, acc);
+    if constexpr (PD)
+    {
+        if constexpr (VS)
+        {
+            double z[5];
+            f_pdiss(C, f, F.terms[KEEP], F.R, F.gamma, z);
+            for (int n = 0; n < 5; ++n) { z[n] += acc[n]; acc[n] = z[n]; }
+        }
+        else f_pdiss(C, f, F.terms[KEEP], F.R, F.gamma, acc);
+    }
+    if constexpr (KEEP)
+    {
+        if constexpr (PD || VS)
+        {
+            double z[5];
+            f_keep(C, f, 2, F.R, F.gamma, z);
+            for (int n = 0; n < 5; ++n) { z[n] += acc[n]; acc[n] = z[n]; }
+        }
+        else f_keep(C, f, 2, F.R, F.gamma, acc);
+    }
+}
+
+template <int KEEP, int PD, int VS>
+__device__ inline void spec_cell(const DevGeom& G, const double* __restrict__ prims, double* __restrict__ rhs,
+                                 const spd_flux_desc& F, int increment, int i, int j, int k, int lb)
+{
+    Cell C{&G, prims + (long long)lb*G.sb, lb};
+    double jac = 1.0;
+    if (!G.identity)
+    {
+        double xi[3]; cell_xi(G, lb, i, j, k, xi);
+        jac = 1.0/(map_deriv(G, 0, xi[0])*map_deriv(G, 1, xi[1])*map_deriv(G, 2, xi[2]));
+    }
+    double relem[5];
+    for (int v = 0; v < 5; ++v) relem[v] = increment ? rhs[cell_index(G, lb, v, i, j, k)] : 0.0;
+    for (int idir = 0; idir < 3; ++idir)
+    {
+        const double inv_dx = 1.0/G.dx[3*lb + idir];
+        for (int pm = 0; pm < 2; ++pm)
+        {
+            FaceIdx f{idir, i, j, k};
+            if (pm == 1) { if (idir == 0) f.i++; else if (idir == 1) f.j++; else f.k++; }
+            double acc[5];
+            spec_face<KEEP, PD, VS>(C, f, F, acc);
+            const double s = jac*(1.0 - 2.0*pm)*inv_dx;
+            for (int n = 0; n < 5; ++n) { acc[n] *= s; relem[n] += acc[n]; }
+        }
+    }
+    for (int v = 0; v < 5; ++v) rhs[cell_index(G, lb, v, i, j, k)] = relem[v];
+}
+
+template <int KEEP, int PD, int VS>
+__global__ void __launch_bounds__(256) __attribute__((flatten))
+k_flux_div_spec(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, spd_flux_desc F, int increment)
+{
+    // same cell mapping as k_flux_div_generic
+    const int tiles_x = (G.nc[0] + 15)/16, tiles_y = (G.nc[1] + 15)/16;
+    long long w = blockIdx.x;
+    const int tx = (int)(w % tiles_x); w /= tiles_x;
+    const int ty = (int)(w % tiles_y); w /= tiles_y;
+    const int k  = (int)(w % G.nc[2]); w /= G.nc[2];
+    const int lb = (int)w;
+    const int i = tx*16 + (threadIdx.x & 15);
+    const int j = ty*16 + (threadIdx.x >> 4);
+    __shared__ DevGeom sG;                // see k_flux_div_generic
+    __shared__ spd_flux_desc sF;
+    if (threadIdx.x == 0) { sG = G; sF = F; }
+    __syncthreads();
+    if (i >= G.nc[0] || j >= G.nc[1]) return;
+    spec_cell<KEEP, PD, VS>(sG, prims, rhs, sF, increment, i, j, k, lb);
+}
+
 } // namespace
+
+int flux_div_spec_mask(const spd_flux_desc& f)
+{
+    if (const char* e = getenv("SPD_DISABLE_SPEC")) if (e[0] == '1') return 0;
+    int mask = 0, stage = 0;
+    for (int m = 0; m < f.nterms; ++m)
+    {
+        const spd_flux_term& t = f.terms[m];
+        if (t.kind == SPD_TERM_KEEP && t.order == 2 && stage < 1) { mask |= 1; stage = 1; }
+        else if (t.kind == SPD_TERM_PDISS && stage < 2) { mask |= 2; stage = 2; }
+        else if (t.kind == SPD_TERM_VISC && stage < 3) { mask |= 4; stage = 3; }
+        else return 0;
+    }
+    return mask;
+}
 
 void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f,
                              bool increment, hipStream_t s)
 {
     const DevGeom& G = g.geom;
     const long long nwg = (long long)((G.nc[0] + 15)/16)*((G.nc[1] + 15)/16)*G.nc[2]*G.nblk;
-    hipLaunchKernelGGL(k_flux_div_generic, dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, increment ? 1 : 0);
+    const int inc = increment ? 1 : 0;
+    switch (flux_div_spec_mask(f))
+    {
+        case 1: hipLaunchKernelGGL((k_flux_div_spec<1, 0, 0>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
+        case 2: hipLaunchKernelGGL((k_flux_div_spec<0, 1, 0>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
+        case 3: hipLaunchKernelGGL((k_flux_div_spec<1, 1, 0>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
+        case 4: hipLaunchKernelGGL((k_flux_div_spec<0, 0, 1>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
+        case 5: hipLaunchKernelGGL((k_flux_div_spec<1, 0, 1>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
+        case 6: hipLaunchKernelGGL((k_flux_div_spec<0, 1, 1>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
+        case 7: hipLaunchKernelGGL((k_flux_div_spec<1, 1, 1>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
+        default:
+            hipLaunchKernelGGL(k_flux_div_generic, dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc);
+    }
     SPD_LAUNCH_CHECK();
 }
 
