@@ -9,7 +9,8 @@
 // for identity coordinates its result equals the reference bit for bit except
 // where a libm routine (pow, tanh) is involved.  The hot combination
 // (KEEP-2 with pressure diffusion and viscous terms) has its own LDS-staged
-// kernel in flux_fast.hip.
+// kernel in flux_fast.hip for identity coordinates, and elsewhere (mapped
+// coordinates) a compile-time specialised form of this kernel, k_flux_div_spec.
 #include "spd_internal.hpp"
 #include <cmath>
 #include <algorithm>
