@@ -377,7 +377,7 @@ int spd_flux_div_kernel(spd_array_t prims, const spd_flux_desc* flux, int* kind)
     {
         require(prims && kind, "null argument");
         validate_flux(flux, prims->a.grid->geom.ng);
-        *kind = flux_div_fast_eligible(*prims->a.grid, *flux) ? 1 : (flux_div_spec_mask(*flux) ? 2 : 0);
+        *kind = flux_div_fast_eligible(*prims->a.grid, *flux) ? 1 : (flux_div_spec_code(*flux) ? 2 : 0);
     });
 }
 

@@ -592,42 +592,59 @@ k_flux_fixup(DevGeom G, const double* __restrict__ prims, double* __restrict__ r
     }
 }
 
-// The per-cell kernel specialised for a composition of cent_keep<2>,
-// pressure_diss_lr and visc_lr (a subsequence of the three, in that order) on
-// any geometry, in particular mapped coordinates, which the LDS-staged kernel
-// does not take.  The terms are known at compile time and the kernel is
-// flattened: every term and helper is inlined, so the small arrays the term
-// functions fill (faces, gradients, metrics) stay in registers instead of the
-// per-lane stack that the run-time dispatch of k_flux_div_generic needs.  The
-// arithmetic is the same code in the same order as generic_cell.
-template <int KEEP, int PD, int VS>
-__device__ inline void spec_face(const Cell& C, const FaceIdx& f, const spd_flux_desc& F, double acc[5])
+// The per-cell kernel specialised for one composition of up to three terms
+// (cent_keep<2> / <4>, totani_lr, pressure_diss_lr, weno_t, fweno_t,
+// first_order_t, visc_lr), on any geometry, in particular mapped coordinates,
+// which the LDS-staged kernel does not take.  The terms are known at compile
+// time and the kernel is flattened: every term and helper is inlined, so the
+// small arrays the term functions fill (stencils, gradients, metrics) stay in
+// registers instead of the per-lane stack that the run-time dispatch of
+// k_flux_div_generic needs.  The arithmetic is the same code in the same order
+// as generic_cell, so the two give the same bits.
+constexpr int TERM_KEEP4 = 9;   // cent_keep<4>; SPD_TERM_KEEP stands for cent_keep<2> here
+
+template <int K>
+__device__ inline void eval_static(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam,
+                                   double out[5])
 {
-    // F0 + (F1 + F2): the last term first, then each earlier one plus the sum
-    if constexpr (VS) f_visc(C, f, F.terms[KEEP + PD], F.R, F.gamma, acc);
-    if constexpr (PD)
-    {
-        if constexpr (VS)
-        {
-            double z[5];
-            f_pdiss(C, f, F.terms[KEEP], F.R, F.gamma, z);
-            for (int n = 0; n < 5; ++n) { z[n] += acc[n]; acc[n] = z[n]; }
-        }
-        else f_pdiss(C, f, F.terms[KEEP], F.R, F.gamma, acc);
-    }
-    if constexpr (KEEP)
-    {
-        if constexpr (PD || VS)
-        {
-            double z[5];
-            f_keep(C, f, 2, F.R, F.gamma, z);
-            for (int n = 0; n < 5; ++n) { z[n] += acc[n]; acc[n] = z[n]; }
-        }
-        else f_keep(C, f, 2, F.R, F.gamma, acc);
-    }
+    if constexpr (K == SPD_TERM_KEEP) f_keep(C, f, 2, R, gam, out);
+    else if constexpr (K == TERM_KEEP4) f_keep(C, f, 4, R, gam, out);
+    else if constexpr (K == SPD_TERM_TOTANI) f_totani(C, f, R, gam, out);
+    else if constexpr (K == SPD_TERM_PDISS) f_pdiss(C, f, t, R, gam, out);
+    else if constexpr (K == SPD_TERM_WENO) f_weno(C, f, t, R, gam, out);
+    else if constexpr (K == SPD_TERM_FWENO) f_fweno(C, f, t, R, gam, out);
+    else if constexpr (K == SPD_TERM_FIRST) f_first(C, f, R, gam, out);
+    else f_visc(C, f, t, R, gam, out);
 }
 
-template <int KEEP, int PD, int VS>
+template <int K>
+__device__ inline void add_static(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam,
+                                  double acc[5])
+{
+    double z[5];
+    eval_static<K>(C, f, t, R, gam, z);
+    for (int n = 0; n < 5; ++n) { z[n] += acc[n]; acc[n] = z[n]; }
+}
+
+// F0 + (F1 + F2): the last term first, then each earlier one plus the sum
+template <int T0, int T1, int T2>
+__device__ inline void spec_face(const Cell& C, const FaceIdx& f, const spd_flux_desc& F, double acc[5])
+{
+    if constexpr (T2 != 0)
+    {
+        eval_static<T2>(C, f, F.terms[2], F.R, F.gamma, acc);
+        add_static<T1>(C, f, F.terms[1], F.R, F.gamma, acc);
+        add_static<T0>(C, f, F.terms[0], F.R, F.gamma, acc);
+    }
+    else if constexpr (T1 != 0)
+    {
+        eval_static<T1>(C, f, F.terms[1], F.R, F.gamma, acc);
+        add_static<T0>(C, f, F.terms[0], F.R, F.gamma, acc);
+    }
+    else eval_static<T0>(C, f, F.terms[0], F.R, F.gamma, acc);
+}
+
+template <int T0, int T1, int T2>
 __device__ inline void spec_cell(const DevGeom& G, const double* __restrict__ prims, double* __restrict__ rhs,
                                  const spd_flux_desc& F, int increment, int i, int j, int k, int lb)
 {
@@ -648,7 +665,7 @@ __device__ inline void spec_cell(const DevGeom& G, const double* __restrict__ pr
             FaceIdx f{idir, i, j, k};
             if (pm == 1) { if (idir == 0) f.i++; else if (idir == 1) f.j++; else f.k++; }
             double acc[5];
-            spec_face<KEEP, PD, VS>(C, f, F, acc);
+            spec_face<T0, T1, T2>(C, f, F, acc);
             const double s = jac*(1.0 - 2.0*pm)*inv_dx;
             for (int n = 0; n < 5; ++n) { acc[n] *= s; relem[n] += acc[n]; }
         }
@@ -656,7 +673,7 @@ __device__ inline void spec_cell(const DevGeom& G, const double* __restrict__ pr
     for (int v = 0; v < 5; ++v) rhs[cell_index(G, lb, v, i, j, k)] = relem[v];
 }
 
-template <int KEEP, int PD, int VS>
+template <int T0, int T1, int T2>
 __global__ void __launch_bounds__(256) __attribute__((flatten))
 k_flux_div_spec(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, spd_flux_desc F, int increment)
 {
@@ -674,24 +691,38 @@ k_flux_div_spec(DevGeom G, const double* __restrict__ prims, double* __restrict_
     if (threadIdx.x == 0) { sG = G; sF = F; }
     __syncthreads();
     if (i >= G.nc[0] || j >= G.nc[1]) return;
-    spec_cell<KEEP, PD, VS>(sG, prims, rhs, sF, increment, i, j, k, lb);
+    spec_cell<T0, T1, T2>(sG, prims, rhs, sF, increment, i, j, k, lb);
 }
 
 } // namespace
 
-int flux_div_spec_mask(const spd_flux_desc& f)
+// Compositions with a specialised kernel, as term codes T0 T1 T2 (0: none).
+#define SPD_SPEC_LIST(X) \
+    X(SPD_TERM_KEEP, 0, 0) X(SPD_TERM_PDISS, 0, 0) X(SPD_TERM_KEEP, SPD_TERM_PDISS, 0) \
+    X(SPD_TERM_VISC, 0, 0) X(SPD_TERM_KEEP, SPD_TERM_VISC, 0) X(SPD_TERM_PDISS, SPD_TERM_VISC, 0) \
+    X(SPD_TERM_KEEP, SPD_TERM_PDISS, SPD_TERM_VISC) \
+    X(TERM_KEEP4, 0, 0) X(SPD_TERM_TOTANI, 0, 0) X(SPD_TERM_WENO, 0, 0) X(SPD_TERM_FWENO, 0, 0) \
+    X(SPD_TERM_FIRST, 0, 0) X(TERM_KEEP4, SPD_TERM_VISC, 0) X(SPD_TERM_TOTANI, SPD_TERM_VISC, 0) \
+    X(SPD_TERM_WENO, SPD_TERM_VISC, 0) X(SPD_TERM_FWENO, SPD_TERM_VISC, 0) X(SPD_TERM_FIRST, SPD_TERM_VISC, 0)
+
+int flux_div_spec_code(const spd_flux_desc& f)
 {
     if (const char* e = getenv("SPD_DISABLE_SPEC")) if (e[0] == '1') return 0;
-    int mask = 0, stage = 0;
+    if (f.nterms < 1 || f.nterms > 3) return 0;
+    int code = 0;
     for (int m = 0; m < f.nterms; ++m)
     {
         const spd_flux_term& t = f.terms[m];
-        if (t.kind == SPD_TERM_KEEP && t.order == 2 && stage < 1) { mask |= 1; stage = 1; }
-        else if (t.kind == SPD_TERM_PDISS && stage < 2) { mask |= 2; stage = 2; }
-        else if (t.kind == SPD_TERM_VISC && stage < 3) { mask |= 4; stage = 3; }
-        else return 0;
+        int k = t.kind;
+        if (k == SPD_TERM_KEEP) { if (t.order == 4) k = TERM_KEEP4; else if (t.order != 2) return 0; }
+        else if (k < SPD_TERM_TOTANI || k > SPD_TERM_VISC) return 0;
+        code = code*10 + k;
     }
-    return mask;
+    for (int m = f.nterms; m < 3; ++m) code *= 10;
+#define SPD_SPEC_MATCH(a, b, c) if (code == (a)*100 + (b)*10 + (c)) return code;
+    SPD_SPEC_LIST(SPD_SPEC_MATCH)
+#undef SPD_SPEC_MATCH
+    return 0;
 }
 
 void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f,
@@ -700,15 +731,14 @@ void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, co
     const DevGeom& G = g.geom;
     const long long nwg = (long long)((G.nc[0] + 15)/16)*((G.nc[1] + 15)/16)*G.nc[2]*G.nblk;
     const int inc = increment ? 1 : 0;
-    switch (flux_div_spec_mask(f))
+    switch (flux_div_spec_code(f))
     {
-        case 1: hipLaunchKernelGGL((k_flux_div_spec<1, 0, 0>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
-        case 2: hipLaunchKernelGGL((k_flux_div_spec<0, 1, 0>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
-        case 3: hipLaunchKernelGGL((k_flux_div_spec<1, 1, 0>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
-        case 4: hipLaunchKernelGGL((k_flux_div_spec<0, 0, 1>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
-        case 5: hipLaunchKernelGGL((k_flux_div_spec<1, 0, 1>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
-        case 6: hipLaunchKernelGGL((k_flux_div_spec<0, 1, 1>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
-        case 7: hipLaunchKernelGGL((k_flux_div_spec<1, 1, 1>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); break;
+#define SPD_SPEC_LAUNCH(a, b, c) \
+        case (a)*100 + (b)*10 + (c): \
+            hipLaunchKernelGGL((k_flux_div_spec<a, b, c>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); \
+            break;
+        SPD_SPEC_LIST(SPD_SPEC_LAUNCH)
+#undef SPD_SPEC_LAUNCH
         default:
             hipLaunchKernelGGL(k_flux_div_generic, dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc);
     }

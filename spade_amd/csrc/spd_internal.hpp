@@ -108,9 +108,8 @@ void launch_flux_div_fixup(const Grid& g, const double* prims, double* rhs, cons
 bool launch_flux_div_fast(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f,
                           bool increment, hipStream_t s);
 bool flux_div_fast_eligible(const Grid& g, const spd_flux_desc& f);
-// non-zero (bit 0 cent_keep<2>, bit 1 pressure_diss_lr, bit 2 visc_lr) when the
-// composition has a compile-time specialised per-cell kernel
-int flux_div_spec_mask(const spd_flux_desc& f);
+// non-zero when the composition has a compile-time specialised per-cell kernel
+int flux_div_spec_code(const spd_flux_desc& f);
 void launch_exchange(const Grid& g, double* a, hipStream_t s);
 void launch_wall_fill(const Grid& g, double* a, int face, double t_wall, hipStream_t s);
 void launch_fill(const Grid& g, double* a, double value, hipStream_t s);
