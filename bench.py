@@ -188,6 +188,48 @@ def rows_measure(sa, g, q, exch, gas, flux, reps=10):
     return out
 
 
+def compositions_measure(sa, blocks=8, reps=10):
+    """flux_div of the compositions outside the LDS-staged kernel's set on the
+    TGV (blocks^3 blocks of 16^3 cells, identity coordinates): the
+    compile-time specialised per-cell kernel against the generic one with
+    run-time term dispatch (SPD_DISABLE_SPEC=1), mean of reps launches after
+    three untimed ones.  The two give the same bits."""
+    nb, nc, ng = (blocks,) * 3, (16, 16, 16), 2
+    L, gamma, R = 2.0 * np.pi, 1.4, 100.0 / 1.4
+    g = sa.CartesianGrid(nc, nb, ((0.0, L),) * 3, num_exch=ng)
+    host = np.zeros(g.host_shape)
+    host[:, ng:-ng, ng:-ng, ng:-ng] = tgv_interior(nb, nc, L, R)
+    q = sa.GridArray(g).upload(host)
+    sa.create_exchange(q).exchange(q)
+    rhs = sa.GridArray(g, fill=0.0)
+    gas = sa.IdealGas(gamma, R)
+    visc = sa.visc_lr(sa.constant_viscosity(1e-3, 0.72), gas)
+    cases = {"weno_rusanov": sa.weno(gas), "fweno": sa.fweno(gas), "first_order": sa.first_order(gas),
+             "totani_lr": sa.totani_lr(gas), "cent_keep4+visc_lr": sa.compose(sa.cent_keep(4, gas), visc),
+             "weno_rusanov+visc_lr": sa.compose(sa.weno(gas), visc)}
+    out = {"workload": f"TGV {blocks * 16}^3, identity coordinates, FP64"}
+    for name, flux in cases.items():
+        row = {}
+        for generic in (False, True):
+            if generic:
+                os.environ["SPD_DISABLE_SPEC"] = "1"
+            try:
+                kind = sa.flux_div_kernel(q, flux)
+                for _ in range(3):
+                    sa.flux_div(q, rhs, flux, increment=False)
+                g.ctx.sync()
+                g.ctx.timer_start()
+                for _ in range(reps):
+                    sa.flux_div(q, rhs, flux, increment=False)
+                ms = g.ctx.timer_stop() / reps
+            finally:
+                os.environ.pop("SPD_DISABLE_SPEC", None)
+            row[kind] = {"ms": round(ms, 4), "mcell_rhs_per_s": round(g.num_cells / (ms * 1e-3) * 1e-6, 1)}
+        row["speedup"] = round(row["generic"]["ms"] / row["specialized"]["ms"], 2)
+        out[name] = row
+    return out
+
+
 def channel_measure(sa, reps=20):
     """flux_div on the stretched wall-bounded channel (192 x 128 x 96 cells,
     12 x 8 x 6 blocks of 16^3, tanh wall-normal map, cent_keep<2> +
@@ -295,11 +337,19 @@ def main():
     ap.add_argument("--study-steps", type=int, default=1000)
     ap.add_argument("--channel", action="store_true",
                     help="instead of the benchmark: time flux_div on the 192x128x96 stretched channel only")
+    ap.add_argument("--compositions", action="store_true",
+                    help="instead of the benchmark: other flux compositions on the 128^3 TGV, specialised "
+                         "vs generic per-cell kernel")
     args = ap.parse_args()
     if args.channel:
         import spade_amd as sa
         sa.load()
         print(json.dumps(channel_measure(sa)), flush=True)
+        return
+    if args.compositions:
+        import spade_amd as sa
+        sa.load()
+        print(json.dumps(compositions_measure(sa)), flush=True)
         return
     if args.precision_study:
         import spade_amd as sa
