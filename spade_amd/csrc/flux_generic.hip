@@ -694,13 +694,45 @@ k_flux_div_spec(DevGeom G, const double* __restrict__ prims, double* __restrict_
     spec_cell<T0, T1, T2>(sG, prims, rhs, sF, increment, i, j, k, lb);
 }
 
+// k_flux_fixup with the specialised per-cell code (the compositions the
+// LDS-staged kernel takes all have one)
+template <int T0, int T1, int T2>
+__global__ void __launch_bounds__(256) __attribute__((flatten))
+k_flux_fixup_spec(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, spd_flux_desc F,
+                  int increment, const int* __restrict__ start)
+{
+    const int nz = G.nc[2];
+    bool any = false;
+    for (int lb = blockIdx.x; lb < G.nblk; lb += gridDim.x) any |= start[lb] < nz;
+    if (!any) return;
+    __shared__ DevGeom sG;
+    __shared__ spd_flux_desc sF;
+    if (threadIdx.x == 0) { sG = G; sF = F; }
+    __syncthreads();
+    const int per_plane = G.nc[0]*G.nc[1];
+    for (int lb = blockIdx.x; lb < G.nblk; lb += gridDim.x)
+    {
+        int s = start[lb];
+        if (s >= nz) continue;
+        if (s < 0) s = 0;
+        const int n = per_plane*(nz - s);
+        for (int c = threadIdx.x; c < n; c += blockDim.x)
+        {
+            const int i = c % G.nc[0], j = (c / G.nc[0]) % G.nc[1], k = s + c/per_plane;
+            spec_cell<T0, T1, T2>(sG, prims, rhs, sF, increment, i, j, k, lb);
+        }
+    }
+}
+
 } // namespace
 
-// Compositions with a specialised kernel, as term codes T0 T1 T2 (0: none).
-#define SPD_SPEC_LIST(X) \
+// Compositions with a specialised kernel, as term codes T0 T1 T2 (0: none);
+// the first list is the LDS-staged kernel's set (its fix-up launches)
+#define SPD_SPEC_FAST_LIST(X) \
     X(SPD_TERM_KEEP, 0, 0) X(SPD_TERM_PDISS, 0, 0) X(SPD_TERM_KEEP, SPD_TERM_PDISS, 0) \
     X(SPD_TERM_VISC, 0, 0) X(SPD_TERM_KEEP, SPD_TERM_VISC, 0) X(SPD_TERM_PDISS, SPD_TERM_VISC, 0) \
-    X(SPD_TERM_KEEP, SPD_TERM_PDISS, SPD_TERM_VISC) \
+    X(SPD_TERM_KEEP, SPD_TERM_PDISS, SPD_TERM_VISC)
+#define SPD_SPEC_LIST(X) SPD_SPEC_FAST_LIST(X) \
     X(TERM_KEEP4, 0, 0) X(SPD_TERM_TOTANI, 0, 0) X(SPD_TERM_WENO, 0, 0) X(SPD_TERM_FWENO, 0, 0) \
     X(SPD_TERM_FIRST, 0, 0) X(TERM_KEEP4, SPD_TERM_VISC, 0) X(SPD_TERM_TOTANI, SPD_TERM_VISC, 0) \
     X(SPD_TERM_WENO, SPD_TERM_VISC, 0) X(SPD_TERM_FWENO, SPD_TERM_VISC, 0) X(SPD_TERM_FIRST, SPD_TERM_VISC, 0)
@@ -748,8 +780,20 @@ void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, co
 void launch_flux_div_fixup(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f, bool increment,
                            hipStream_t s)
 {
-    hipLaunchKernelGGL(k_flux_fixup, dim3(std::min(g.geom.nblk, 1024)), dim3(256), 0, s, g.geom, prims, rhs, f,
-                       increment ? 1 : 0, g.d_start);
+    const dim3 grid((unsigned)std::min(g.geom.nblk, 1024));
+    const int inc = increment ? 1 : 0;
+    switch (flux_div_spec_code(f))
+    {
+#define SPD_FIXUP_LAUNCH(a, b, c) \
+        case (a)*100 + (b)*10 + (c): \
+            hipLaunchKernelGGL((k_flux_fixup_spec<a, b, c>), grid, dim3(256), 0, s, g.geom, prims, rhs, f, inc, \
+                               g.d_start); \
+            break;
+        SPD_SPEC_FAST_LIST(SPD_FIXUP_LAUNCH)
+#undef SPD_FIXUP_LAUNCH
+        default:
+            hipLaunchKernelGGL(k_flux_fixup, grid, dim3(256), 0, s, g.geom, prims, rhs, f, inc, g.d_start);
+    }
     SPD_LAUNCH_CHECK();
 }
 

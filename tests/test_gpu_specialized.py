@@ -74,3 +74,37 @@ def test_specialized_equals_generic(geometry, name, terms):
         assert (ks, kg) == ("specialized", "generic")
         assert np.all(np.isfinite(spec))
         assert np.array_equal(spec, gen), np.max(np.abs(spec - gen))
+
+
+@pytest.mark.parametrize("increment", [False, True], ids=["overwrite", "increment"])
+def test_fixup_specialized_equals_generic(increment):
+    """The fix-up launch after the LDS-staged kernel (blocks whose face
+    temperatures leave the power law's series range) uses the specialised
+    per-cell code too: same bits as with the generic per-cell code."""
+    from test_gpu_fast import _state as fast_state
+    sa = _sa()
+    nb, nc, ng = (2, 2, 2), (16, 16, 16), 2
+    host, L = fast_state(nb, nc, ng, 11)
+    inner = host[:, ng:-ng, ng:-ng, ng:-ng]
+    inner[..., 1] = 1.0 + 0.005 * np.random.default_rng(3).standard_normal(inner.shape[:-1])
+    inner[1, 5, 3, 4, 1] = 1.4
+    inner[5, :, :, :, 1] = 1.25
+    g = sa.CartesianGrid(nc, nb, ((0.0, L),) * 3, num_exch=ng)
+    q = sa.GridArray(g).upload(host)
+    sa.create_exchange(q).exchange(q)
+    terms = [dict(kind=1, order=2), VISC[1]]
+    init = np.random.default_rng(5).standard_normal(g.host_shape)
+    out = []
+    for generic in (False, True):
+        rhs = sa.GridArray(g).upload(init)
+        if generic:
+            os.environ["SPD_DISABLE_SPEC"] = "1"
+        try:
+            assert sa.flux_div_kernel(q, RawFlux(terms, GAMMA, R)) == "fast"
+            sa.flux_div(q, rhs, RawFlux(terms, GAMMA, R), increment=increment)
+        finally:
+            os.environ.pop("SPD_DISABLE_SPEC", None)
+        out.append(rhs.download())
+    print("fixup", increment, np.max(np.abs(out[0] - out[1])), flush=True)
+    assert np.all(np.isfinite(out[0]))
+    assert np.array_equal(out[0], out[1])
