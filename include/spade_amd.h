@@ -91,7 +91,10 @@ enum {
     SPD_TERM_WENO,        /* weno_t<rusanov|phys_flux>   convective.h:255-333, flux_funcs.h */
     SPD_TERM_FWENO,       /* fweno_t                     convective.h:335-497 */
     SPD_TERM_FIRST,       /* first_order_t (rusanov)     convective.h:14-33 */
-    SPD_TERM_VISC         /* visc_lr                     navier-stokes/viscous.h:13-113 */
+    SPD_TERM_VISC,        /* visc_lr                     navier-stokes/viscous.h:13-113 */
+    SPD_TERM_HYBRID       /* hybrid_scheme_t: cent_keep<order> blended with weno_t / fweno_t by a face
+                             sensor (navier-stokes/hybrid_scheme.h:14-48, state_sensor.h:21-60); runs on
+                             compile-time specialised kernels only, alone or followed by visc_lr */
 };
 enum { SPD_SENSOR_CONST = 0, SPD_SENSOR_DUCROS = 1 };   /* state_sensor.h:21-60 */
 enum { SPD_SPLIT_RUSANOV = 0, SPD_SPLIT_PHYS = 1 };     /* flux_funcs.h:8-126 */
@@ -108,6 +111,11 @@ typedef struct {
     int    smooth;                   /* WENO/FWENO: 1 = nonlinear weights (enable_smooth) */
     int    law;                      /* VISC */
     double mu, T_ref, power, prandtl;/* VISC: constant mu, or power law mu_ref (T/T_ref)^power */
+    int    diss_kind;                /* HYBRID: dissipative scheme, SPD_TERM_WENO or SPD_TERM_FWENO (with
+                                        split / smooth as for those terms); order = the central cent_keep order;
+                                        sensor / sensor_param = the blending sensor */
+    int    full;                     /* HYBRID: 1 = full_flux ((1 - b) F_central + b F_diss),
+                                        0 = diss_flux (F_central + b F_diss) (hybrid_scheme.h:7-12, 36-41) */
 } spd_flux_term;
 
 typedef struct {

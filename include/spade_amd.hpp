@@ -148,6 +148,35 @@ inline flux fweno(const ideal_gas_t& gas, bool smooth = true) { auto t = term(SP
 // convective.h:14-33
 inline flux first_order(const ideal_gas_t& gas) { return flux(term(SPD_TERM_FIRST), gas); }
 
+// hybrid_scheme_t(scheme0, scheme1, blender, full_flux | diss_flux) (navier-stokes/hybrid_scheme.h:14-48):
+// scheme0 = cent_keep<2|4>, scheme1 = weno(...) or fweno(...)
+enum hybrid_tag { diss_flux = 0, full_flux = 1 };
+inline spd_flux_term hybrid_term(const flux& central, const flux& diss, hybrid_tag tag)
+{
+    const spd_flux_desc& c = central.desc();
+    const spd_flux_desc& d = diss.desc();
+    if (c.nterms != 1 || c.terms[0].kind != SPD_TERM_KEEP)
+        throw error(SPD_ERR_ARG, "hybrid_scheme_t: scheme0 must be cent_keep");
+    if (d.nterms != 1 || (d.terms[0].kind != SPD_TERM_WENO && d.terms[0].kind != SPD_TERM_FWENO))
+        throw error(SPD_ERR_ARG, "hybrid_scheme_t: scheme1 must be weno or fweno");
+    spd_flux_term t = d.terms[0];
+    t.diss_kind = d.terms[0].kind;
+    t.kind = SPD_TERM_HYBRID;
+    t.order = c.terms[0].order;
+    t.full = tag;
+    return t;
+}
+inline flux hybrid_scheme_t(const flux& central, const flux& diss, const const_sensor_t& s, hybrid_tag tag)
+{
+    auto t = hybrid_term(central, diss, tag); t.sensor = SPD_SENSOR_CONST; t.sensor_param = s.alpha;
+    return flux(t, ideal_gas_t{central.desc().gamma, central.desc().R});
+}
+inline flux hybrid_scheme_t(const flux& central, const flux& diss, const ducros_t& s, hybrid_tag tag)
+{
+    auto t = hybrid_term(central, diss, tag); t.sensor = SPD_SENSOR_DUCROS; t.sensor_param = s.epsilon;
+    return flux(t, ideal_gas_t{central.desc().gamma, central.desc().R});
+}
+
 // viscous_laws.h:62-100 and 102-135
 struct constant_viscosity_t { double mu, prandtl; };
 struct power_law_t { double mu_ref, T_ref, power, prandtl; };

@@ -261,6 +261,27 @@ def first_order(gas):
     return _Term(gas, kind=_abi.TERM_FIRST)
 
 
+# hybrid_scheme_t flux tags (navier-stokes/hybrid_scheme.h:7-12)
+full_flux, diss_flux = 1, 0
+
+
+def hybrid_scheme(central, dissipative, sensor, flux_tag=full_flux):
+    """hybrid_scheme_t(scheme0, scheme1, blender, tag) (navier-stokes/hybrid_scheme.h:14-48):
+    cent_keep<2|4> blended with weno(...) or fweno(...) by the face sensor b
+    (const_sensor or ducros); full_flux = (1 - b) F0 + b F1, diss_flux = F0 + b F1."""
+    c, d = central.terms(), dissipative.terms()
+    if len(c) != 1 or c[0].fields.get("kind") != _abi.TERM_KEEP:
+        raise ValueError("hybrid_scheme: the central scheme must be cent_keep(order, gas)")
+    if len(d) != 1 or d[0].fields.get("kind") not in (_abi.TERM_WENO, _abi.TERM_FWENO):
+        raise ValueError("hybrid_scheme: the dissipative scheme must be weno(...) or fweno(...)")
+    if c[0].gas.gamma != d[0].gas.gamma or c[0].gas.R != d[0].gas.R:
+        raise ValueError("hybrid_scheme: both schemes must use the same gas")
+    df = d[0].fields
+    return _Term(c[0].gas, kind=_abi.TERM_HYBRID, order=c[0].fields["order"], diss_kind=df["kind"],
+                 split=df.get("split", _abi.SPLIT_RUSANOV), smooth=df.get("smooth", 1), sensor=sensor.kind,
+                 sensor_param=sensor.param, full=int(flux_tag))
+
+
 class _Law:
     def __init__(self, **fields):
         self.fields = fields

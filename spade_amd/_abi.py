@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(HERE, "libspade_amd.so")
 
 SPD_OK, SPD_ERR_ARG, SPD_ERR_HIP, SPD_ERR_UNSUPPORTED, SPD_ERR_MEMORY, SPD_ERR_IO = range(6)
 MAP_IDENTITY, MAP_TANH = 0, 1
-TERM_KEEP, TERM_TOTANI, TERM_PDISS, TERM_WENO, TERM_FWENO, TERM_FIRST, TERM_VISC = range(1, 8)
+TERM_KEEP, TERM_TOTANI, TERM_PDISS, TERM_WENO, TERM_FWENO, TERM_FIRST, TERM_VISC, TERM_HYBRID = range(1, 9)
 SENSOR_CONST, SENSOR_DUCROS = 0, 1
 SPLIT_RUSANOV, SPLIT_PHYS = 0, 1
 LAW_CONST, LAW_POWER = 0, 1
@@ -35,7 +35,7 @@ class FluxTerm(C.Structure):
                 ("sensor_param", C.c_double), ("eps_p", C.c_double), ("eps_T", C.c_double),
                 ("split", C.c_int), ("smooth", C.c_int), ("law", C.c_int),
                 ("mu", C.c_double), ("T_ref", C.c_double), ("power", C.c_double),
-                ("prandtl", C.c_double)]
+                ("prandtl", C.c_double), ("diss_kind", C.c_int), ("full", C.c_int)]
 
 
 class FluxDesc(C.Structure):

@@ -62,10 +62,25 @@ void validate_flux(const spd_flux_desc* f, int ng)
             case SPD_TERM_TOTANI: case SPD_TERM_FIRST:
                 require(ng >= 1, "flux needs at least one ghost layer");
                 break;
+            case SPD_TERM_HYBRID:
+                require(t.order == 2 || t.order == 4, "hybrid central scheme must be cent_keep<2> or cent_keep<4>");
+                require(t.diss_kind == SPD_TERM_WENO || t.diss_kind == SPD_TERM_FWENO,
+                        "hybrid dissipative scheme must be SPD_TERM_WENO or SPD_TERM_FWENO");
+                require(ng >= 2, "hybrid WENO stencils need 2 ghost layers");
+                require(t.diss_kind == SPD_TERM_FWENO || t.split == SPD_SPLIT_RUSANOV || t.split == SPD_SPLIT_PHYS,
+                        "WENO split must be SPD_SPLIT_RUSANOV or SPD_SPLIT_PHYS");
+                require(t.sensor == SPD_SENSOR_CONST || t.sensor == SPD_SENSOR_DUCROS,
+                        "hybrid sensor must be SPD_SENSOR_CONST or SPD_SENSOR_DUCROS");
+                require(t.full == 0 || t.full == 1, "hybrid full must be 0 (diss_flux) or 1 (full_flux)");
+                break;
             default:
                 throw ArgError{"unknown flux term kind " + std::to_string(t.kind)};
         }
     }
+    // hybrid_scheme_t runs only on compile-time specialised kernels (no
+    // run-time term dispatch): alone, or followed by visc_lr
+    if (flux_has_hybrid(*f) && !flux_div_spec_code(*f))
+        throw UnsupportedError{"hybrid_scheme_t is supported alone or composed as hybrid + visc_lr"};
 }
 
 // Host arrays in the reference layout (AoS, variable fastest) cross the

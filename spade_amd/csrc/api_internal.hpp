@@ -10,6 +10,8 @@ namespace spd {
 
 // a file could not be opened, read or written
 struct IoError { std::string msg; };
+// a valid request the library does not implement
+struct UnsupportedError { std::string msg; };
 
 void set_error(const std::string& msg);
 
@@ -26,6 +28,7 @@ template <typename F> int guarded(F&& fn)
     }
     catch (const ArgError& e) { set_error(e.msg); return SPD_ERR_ARG; }
     catch (const IoError& e) { set_error(e.msg); return SPD_ERR_IO; }
+    catch (const UnsupportedError& e) { set_error(e.msg); return SPD_ERR_UNSUPPORTED; }
     catch (const HipError& e)
     {
         set_error(std::string("HIP error ") + hipGetErrorString(e.code) + " in " + e.where);

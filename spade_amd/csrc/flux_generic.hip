@@ -155,7 +155,7 @@ __device__ inline double cfd_coeff(int idx, int order)
 }
 
 // totani_lr (navier-stokes/convective.h:68-93)
-__device__ void f_totani(const Cell& C, const FaceIdx& f, double R, double gam, double out[5])
+__device__ __forceinline__ void f_totani(const Cell& C, const FaceIdx& f, double R, double gam, double out[5])
 {
     const Prim ql = cell_at(C, f, -1), qr = cell_at(C, f, 1);
     double nl[3], nr[3]; cell_metric(C, f, -1, nl); cell_metric(C, f, 1, nr);
@@ -174,7 +174,7 @@ __device__ void f_totani(const Cell& C, const FaceIdx& f, double R, double gam, 
 }
 
 // cent_keep_scheme_t<order> (convective.h:117-184)
-__device__ void f_keep(const Cell& C, const FaceIdx& f, int order, double R, double gam, double out[5])
+__device__ __forceinline__ void f_keep(const Cell& C, const FaceIdx& f, int order, double R, double gam, double out[5])
 {
     const int hw = order/2;
     double nface[3]; face_metric(C, f, nface);
@@ -228,7 +228,7 @@ __device__ inline double ducros(const double gr[3][5], double eps)
 }
 
 // pressure_diss_lr (convective.h:212-245)
-__device__ void f_pdiss(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam_, double out[5])
+__device__ __forceinline__ void f_pdiss(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam_, double out[5])
 {
     const Prim ql = cell_at(C, f, -1), qr = cell_at(C, f, 1);
     const double delta_p = ql.p - qr.p;
@@ -302,7 +302,7 @@ __device__ inline void split_phys(double R, double gam, const Prim& q, const dou
 }
 
 // weno_t (convective.h:275-332)
-__device__ void f_weno(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam, double out[5])
+__device__ __forceinline__ void f_weno(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam, double out[5])
 {
     double fu[4][5], fd[4][5];
     for (int c = 0; c < 4; ++c)
@@ -344,7 +344,7 @@ __device__ void f_weno(const Cell& C, const FaceIdx& f, const spd_flux_term& t, 
 }
 
 // fweno_t (convective.h:355-496)
-__device__ void f_fweno(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam, double out[5])
+__device__ __forceinline__ void f_fweno(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam, double out[5])
 {
     Prim q[4]; double nv[4][3];
     double rho[4], hsr[4], fluxv[4], dissv[4];
@@ -429,7 +429,7 @@ __device__ void f_fweno(const Cell& C, const FaceIdx& f, const spd_flux_term& t,
 }
 
 // first_order_t (convective.h:27-32)
-__device__ void f_first(const Cell& C, const FaceIdx& f, double R, double gam, double out[5])
+__device__ __forceinline__ void f_first(const Cell& C, const FaceIdx& f, double R, double gam, double out[5])
 {
     double fuL[5], fdL[5], fuR[5], fdR[5], nl[3], nr[3];
     cell_metric(C, f, -1, nl); cell_metric(C, f, 1, nr);
@@ -440,7 +440,7 @@ __device__ void f_first(const Cell& C, const FaceIdx& f, double R, double gam, d
 
 // visc_lr (viscous.h:39-112) with constant_viscosity_t or the power law
 // (viscous_laws.h:62-135)
-__device__ void f_visc(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam, double out[5])
+__device__ __forceinline__ void f_visc(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam, double out[5])
 {
     const Prim ql = cell_at(C, f, -1), qr = cell_at(C, f, 1);
     Prim qf;
@@ -484,7 +484,39 @@ __device__ void f_visc(const Cell& C, const FaceIdx& f, const spd_flux_term& t, 
     out[4] = -(n[0]*tau[2][0] + n[1]*tau[2][1] + n[2]*tau[2][2]);
 }
 
-__device__ void eval_term(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam, double out[5])
+// hybrid_scheme_t (navier-stokes/hybrid_scheme.h:30-44): the sensor b of the
+// face (const_sensor_t, or ducros_t from the face gradient, state_sensor.h:
+// 21-60), then the central cent_keep<ORDER> flux F0 and the dissipative
+// weno_t / fweno_t flux F1; full_flux: (1 - b) F0 + b F1, diss_flux: F0 + b F1,
+// each product rounded before the sum, as the reference's f0 *= c0; f1 *= c1;
+// f0 += f1.  Only instantiated in the compile-time specialised kernels, where
+// it is inlined like every other term.
+template <int ORDER, int DISS>
+__device__ inline void f_hybrid(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam,
+                                double out[5])
+{
+    double blend = t.sensor_param;
+    if (t.sensor == SPD_SENSOR_DUCROS)
+    {
+        double gr[3][5]; face_grad(C, f, gr);
+        blend = ducros(gr, t.sensor_param);
+    }
+    double f0[5], f1[5];
+    f_keep(C, f, ORDER, R, gam, f0);
+    if constexpr (DISS == SPD_TERM_FWENO) f_fweno(C, f, t, R, gam, f1);
+    else                                  f_weno(C, f, t, R, gam, f1);
+    double coeff0 = 1.0 - blend;
+    const double coeff1 = blend;
+    if (!t.full) coeff0 = 1.0;
+    for (int n = 0; n < 5; ++n) f0[n] *= coeff0;
+    for (int n = 0; n < 5; ++n) f1[n] *= coeff1;
+    for (int n = 0; n < 5; ++n) out[n] = f0[n] + f1[n];
+}
+
+// The run-time term dispatch of the generic kernel.  SPD_TERM_HYBRID is not
+// among its cases: a hybrid composition is accepted only where a specialised
+// kernel exists for it (flux_div_spec_code), and refused otherwise.
+__device__ __forceinline__ void eval_term(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam, double out[5])
 {
     switch (t.kind)
     {
@@ -536,7 +568,7 @@ __device__ inline void generic_cell(const DevGeom& G, const double* __restrict__
     for (int v = 0; v < 5; ++v) rhs[cell_index(G, lb, v, i, j, k)] = relem[v];
 }
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((flatten))
 k_flux_div_generic(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, spd_flux_desc F, int increment)
 {
     const int tiles_x = (G.nc[0] + 15)/16, tiles_y = (G.nc[1] + 15)/16;
@@ -562,7 +594,7 @@ k_flux_div_generic(DevGeom G, const double* __restrict__ prims, double* __restri
 // temperature outside its pow series range): planes start[lb] .. nz-1 of the
 // block, one workgroup per block.  Planes below start[lb] are complete and
 // planes from start[lb] on were not written, so increment mode stays exact.
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256) __attribute__((flatten))
 k_flux_fixup(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, spd_flux_desc F, int increment,
              const int* __restrict__ start)
 {
@@ -602,6 +634,8 @@ k_flux_fixup(DevGeom G, const double* __restrict__ prims, double* __restrict__ r
 // k_flux_div_generic needs.  The arithmetic is the same code in the same order
 // as generic_cell, so the two give the same bits.
 constexpr int TERM_KEEP4 = 9;   // cent_keep<4>; SPD_TERM_KEEP stands for cent_keep<2> here
+// hybrid_scheme_t codes: central cent_keep order x dissipative scheme
+constexpr int TERM_HYB2W = 21, TERM_HYB2F = 22, TERM_HYB4W = 23, TERM_HYB4F = 24;
 
 template <int K>
 __device__ inline void eval_static(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam,
@@ -614,6 +648,10 @@ __device__ inline void eval_static(const Cell& C, const FaceIdx& f, const spd_fl
     else if constexpr (K == SPD_TERM_WENO) f_weno(C, f, t, R, gam, out);
     else if constexpr (K == SPD_TERM_FWENO) f_fweno(C, f, t, R, gam, out);
     else if constexpr (K == SPD_TERM_FIRST) f_first(C, f, R, gam, out);
+    else if constexpr (K == TERM_HYB2W) f_hybrid<2, SPD_TERM_WENO>(C, f, t, R, gam, out);
+    else if constexpr (K == TERM_HYB2F) f_hybrid<2, SPD_TERM_FWENO>(C, f, t, R, gam, out);
+    else if constexpr (K == TERM_HYB4W) f_hybrid<4, SPD_TERM_WENO>(C, f, t, R, gam, out);
+    else if constexpr (K == TERM_HYB4F) f_hybrid<4, SPD_TERM_FWENO>(C, f, t, R, gam, out);
     else f_visc(C, f, t, R, gam, out);
 }
 
@@ -735,11 +773,22 @@ k_flux_fixup_spec(DevGeom G, const double* __restrict__ prims, double* __restric
 #define SPD_SPEC_LIST(X) SPD_SPEC_FAST_LIST(X) \
     X(TERM_KEEP4, 0, 0) X(SPD_TERM_TOTANI, 0, 0) X(SPD_TERM_WENO, 0, 0) X(SPD_TERM_FWENO, 0, 0) \
     X(SPD_TERM_FIRST, 0, 0) X(TERM_KEEP4, SPD_TERM_VISC, 0) X(SPD_TERM_TOTANI, SPD_TERM_VISC, 0) \
-    X(SPD_TERM_WENO, SPD_TERM_VISC, 0) X(SPD_TERM_FWENO, SPD_TERM_VISC, 0) X(SPD_TERM_FIRST, SPD_TERM_VISC, 0)
+    X(SPD_TERM_WENO, SPD_TERM_VISC, 0) X(SPD_TERM_FWENO, SPD_TERM_VISC, 0) X(SPD_TERM_FIRST, SPD_TERM_VISC, 0) \
+    X(TERM_HYB2W, 0, 0) X(TERM_HYB2F, 0, 0) X(TERM_HYB4W, 0, 0) X(TERM_HYB4F, 0, 0) \
+    X(TERM_HYB2W, SPD_TERM_VISC, 0) X(TERM_HYB2F, SPD_TERM_VISC, 0) X(TERM_HYB4W, SPD_TERM_VISC, 0) \
+    X(TERM_HYB4F, SPD_TERM_VISC, 0)
+
+bool flux_has_hybrid(const spd_flux_desc& f)
+{
+    for (int m = 0; m < f.nterms && m < SPD_MAX_TERMS; ++m) if (f.terms[m].kind == SPD_TERM_HYBRID) return true;
+    return false;
+}
 
 int flux_div_spec_code(const spd_flux_desc& f)
 {
-    if (const char* e = getenv("SPD_DISABLE_SPEC")) if (e[0] == '1') return 0;
+    // SPD_DISABLE_SPEC=1 sends the other compositions to the generic kernel
+    // (for timing comparisons); a hybrid composition has no generic form
+    if (!flux_has_hybrid(f)) if (const char* e = getenv("SPD_DISABLE_SPEC")) if (e[0] == '1') return 0;
     if (f.nterms < 1 || f.nterms > 3) return 0;
     int code = 0;
     for (int m = 0; m < f.nterms; ++m)
@@ -747,11 +796,18 @@ int flux_div_spec_code(const spd_flux_desc& f)
         const spd_flux_term& t = f.terms[m];
         int k = t.kind;
         if (k == SPD_TERM_KEEP) { if (t.order == 4) k = TERM_KEEP4; else if (t.order != 2) return 0; }
+        else if (k == SPD_TERM_HYBRID)
+        {
+            if (t.order != 2 && t.order != 4) return 0;
+            if (t.diss_kind == SPD_TERM_WENO) k = t.order == 2 ? TERM_HYB2W : TERM_HYB4W;
+            else if (t.diss_kind == SPD_TERM_FWENO) k = t.order == 2 ? TERM_HYB2F : TERM_HYB4F;
+            else return 0;
+        }
         else if (k < SPD_TERM_TOTANI || k > SPD_TERM_VISC) return 0;
-        code = code*10 + k;
+        code = code*100 + k;
     }
-    for (int m = f.nterms; m < 3; ++m) code *= 10;
-#define SPD_SPEC_MATCH(a, b, c) if (code == (a)*100 + (b)*10 + (c)) return code;
+    for (int m = f.nterms; m < 3; ++m) code *= 100;
+#define SPD_SPEC_MATCH(a, b, c) if (code == (a)*10000 + (b)*100 + (c)) return code;
     SPD_SPEC_LIST(SPD_SPEC_MATCH)
 #undef SPD_SPEC_MATCH
     return 0;
@@ -766,12 +822,13 @@ void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, co
     switch (flux_div_spec_code(f))
     {
 #define SPD_SPEC_LAUNCH(a, b, c) \
-        case (a)*100 + (b)*10 + (c): \
+        case (a)*10000 + (b)*100 + (c): \
             hipLaunchKernelGGL((k_flux_div_spec<a, b, c>), dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc); \
             break;
         SPD_SPEC_LIST(SPD_SPEC_LAUNCH)
 #undef SPD_SPEC_LAUNCH
         default:
+            if (flux_has_hybrid(f)) throw ArgError{"hybrid_scheme_t runs only alone or followed by visc_lr"};
             hipLaunchKernelGGL(k_flux_div_generic, dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc);
     }
     SPD_LAUNCH_CHECK();
@@ -785,7 +842,7 @@ void launch_flux_div_fixup(const Grid& g, const double* prims, double* rhs, cons
     switch (flux_div_spec_code(f))
     {
 #define SPD_FIXUP_LAUNCH(a, b, c) \
-        case (a)*100 + (b)*10 + (c): \
+        case (a)*10000 + (b)*100 + (c): \
             hipLaunchKernelGGL((k_flux_fixup_spec<a, b, c>), grid, dim3(256), 0, s, g.geom, prims, rhs, f, inc, \
                                g.d_start); \
             break;

@@ -6,6 +6,17 @@
 namespace spd {
 namespace {
 
+// max that passes NaN through from either operand.  The reference's
+// utils::max (t1 > t2 ? t1 : t2, core/utils.h:87-90) returns a NaN second
+// operand, and its driver stops on isnan(umax)
+// (development/cuda-tgv/main.cc:251); here the result is NaN whenever any
+// cell's wave speed is, whatever the reduction order, so a diverged state
+// gives a NaN wave speed (fmax would return the max of the finite cells)
+__device__ inline double max_nan(double a, double b)
+{
+    return (a > b || a != a) ? a : b;
+}
+
 // Ghost exchange (grid/exchange_pattern.h:382-424): every ghost cell copies
 // the cell of the neighbour block that owns it (26-neighbour table built on
 // the host, grid/cartesian_blocks.h:73-105; region (ex+1) + 3(ey+1) + 9(ez+1),
@@ -184,13 +195,13 @@ k_max_wavespeed(DevGeom G, const double* __restrict__ q, double gamma, double R,
         const double v = q[cell_index(G, lb, 3, i, j, k)];
         const double w = q[cell_index(G, lb, 4, i, j, k)];
         const double s = sqrt(gamma*R*T) + sqrt(u*u + v*v + w*w);
-        m = fmax(m, s);
+        m = max_nan(s, m);
     }
     red[threadIdx.x] = m;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1)
     {
-        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        if (threadIdx.x < w) red[threadIdx.x] = max_nan(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
     if (threadIdx.x == 0) part[blockIdx.y*gridDim.x + blockIdx.x] = red[0];
@@ -246,12 +257,12 @@ k_max_final(const double* __restrict__ part, int n, double* __restrict__ out)
 {
     __shared__ double red[1024];
     double m = 0.0;
-    for (int t = threadIdx.x; t < n; t += blockDim.x) m = fmax(m, part[t]);
+    for (int t = threadIdx.x; t < n; t += blockDim.x) m = max_nan(part[t], m);
     red[threadIdx.x] = m;
     __syncthreads();
     for (int w = 512; w > 0; w >>= 1)
     {
-        if (threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+        if (threadIdx.x < w) red[threadIdx.x] = max_nan(red[threadIdx.x], red[threadIdx.x + w]);
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = red[0];
@@ -326,15 +337,27 @@ void launch_fill(const Grid& g, double* a, double value, hipStream_t s)
     SPD_LAUNCH_CHECK();
 }
 
+namespace {
+
+// max(|u| + c) over the interior: one workgroup of 1024 per block writes one
+// partial per block into part[0 .. nblk), then one workgroup writes their max
+// to out (device memory)
+void launch_max_wavespeed(const DevGeom& G, const double* q, double gamma, double R, double* part, double* out,
+                          hipStream_t s)
+{
+    hipLaunchKernelGGL(k_max_wavespeed, dim3(1, G.nblk), dim3(1024), 0, s, G, q, gamma, R, part);
+    SPD_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(1024), 0, s, part, G.nblk, out);
+    SPD_LAUNCH_CHECK();
+}
+
+} // namespace
+
 double reduce_max_wavespeed(Grid& g, const double* q, double gamma, double R, hipStream_t s)
 {
     const DevGeom& G = g.geom;
     ensure_red(g);
-    // one workgroup of 1024 per block: one partial per block, then their max on the device
-    hipLaunchKernelGGL(k_max_wavespeed, dim3(1, G.nblk), dim3(1024), 0, s, G, q, gamma, R, g.d_red);
-    SPD_LAUNCH_CHECK();
-    hipLaunchKernelGGL(k_max_final, dim3(1), dim3(1024), 0, s, g.d_red, G.nblk, g.d_red + 2*(size_t)G.nblk);
-    SPD_LAUNCH_CHECK();
+    launch_max_wavespeed(G, q, gamma, R, g.d_red, g.d_red + 2*(size_t)G.nblk, s);
     SPD_HIP(hipMemcpyAsync(g.h_red, g.d_red + 2*(size_t)G.nblk, sizeof(double), hipMemcpyDeviceToHost, s));
     SPD_HIP(hipStreamSynchronize(s));
     return g.h_red[0];

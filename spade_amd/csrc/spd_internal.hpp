@@ -110,6 +110,8 @@ bool launch_flux_div_fast(const Grid& g, const double* prims, double* rhs, const
 bool flux_div_fast_eligible(const Grid& g, const spd_flux_desc& f);
 // non-zero when the composition has a compile-time specialised per-cell kernel
 int flux_div_spec_code(const spd_flux_desc& f);
+// the composition contains a hybrid_scheme_t term
+bool flux_has_hybrid(const spd_flux_desc& f);
 void launch_exchange(const Grid& g, double* a, hipStream_t s);
 void launch_wall_fill(const Grid& g, double* a, int face, double t_wall, hipStream_t s);
 void launch_fill(const Grid& g, double* a, double value, hipStream_t s);

@@ -141,16 +141,20 @@ def fp64_rate(spec, steps, warmup=3):
     return spec.num_cells * steps, ms * 1e-3
 
 
-def serving(n=60, rate=8.0, seed=7, max_chunk=8, budget_gb=64):
-    def make(rng):
-        kind = "tgv" if rng.random() < 0.5 else "vortex"
-        cells = tuple(int(c) for c in rng.choice([32, 64, 96, 128], size=3))
-        mach = float(rng.uniform(0.1, 0.5))
-        re = float(math.exp(rng.uniform(math.log(100.0), math.log(5000.0))))
-        steps = int(rng.integers(100, 2001))
-        return CaseSpec(kind, cells, mach=mach, reynolds=re, steps=steps)
+def serving_spec(rng):
+    """One request of the served stream: TGV or isentropic vortex by coin flip,
+    cells per axis from {32, 64, 96, 128}, Mach ~ U[0.1, 0.5], Re log-uniform
+    in [100, 5000], steps uniform in [100, 2000]."""
+    kind = "tgv" if rng.random() < 0.5 else "vortex"
+    cells = tuple(int(c) for c in rng.choice([32, 64, 96, 128], size=3))
+    mach = float(rng.uniform(0.1, 0.5))
+    re = float(math.exp(rng.uniform(math.log(100.0), math.log(5000.0))))
+    steps = int(rng.integers(100, 2001))
+    return CaseSpec(kind, cells, mach=mach, reynolds=re, steps=steps)
 
-    reqs = poisson_requests(n, rate, make, seed=seed)
+
+def serving(n=60, rate=8.0, seed=7, max_chunk=8, budget_gb=64):
+    reqs = poisson_requests(n, rate, serving_spec, seed=seed)
     b = Batch(int(budget_gb) << 30)
     srv = Server(b, max_chunk=max_chunk, keep_state=False)
     res = srv.run(reqs)

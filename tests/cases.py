@@ -43,13 +43,5 @@ FLUX_CASES = {
 # round-off, not bit for bit, on the GPU (device pow vs glibc pow)
 USES_POW = {"visc_pow", "keep2_visc_pow"}
 
-# compositions the CPU oracle reproduces bit for bit against the reference but
-# the GPU library does not offer: its first hybrid_scheme_t kernel build was
-# followed by a GPU memory fault (in an unrelated launch) whose cause is not
-# established, so the GPU side was taken out and these cases are skipped there
-GPU_UNSUPPORTED = {k for k in FLUX_CASES if k.startswith("hybrid")}
-GPU_UNSUPPORTED_REASON = ("hybrid_scheme_t is pinned on the CPU oracle only; its GPU path was removed after a "
-                          "memory fault of unestablished cause")
-
 G1 = dict(nb=(2, 3, 2), nc=(4, 8, 4), ng=2)
 G2 = dict(nb=(2, 2, 2), nc=(8, 8, 8), ng=2)

@@ -173,3 +173,38 @@ def test_full_ensemble_conserves_and_equals_alone_runs():
     for n in (0, 150, 299):
         alone, _ = _run_alone(specs[n], 10)
         assert np.array_equal(alone, b.download(ids[n])), n
+
+
+def test_served_stream_at_its_sizes_equals_alone_runs():
+    """The served form at the sizes its request stream draws (cells per axis
+    from {32, 64, 96, 128}, 100-2000 steps, Mach 0.1-0.5, Re 100-5000): eight
+    requests from the stream's own sampler, the first forced to 128^3, arriving
+    as a Poisson stream fast enough that they overlap on the device (continuous
+    batching: admitted while others run, retired as they finish).  Every served
+    result -- final conserved fields and the per-step kinetic-energy and
+    enstrophy history -- equals the same case run alone, bit for bit."""
+    from spade_amd import studies
+    from spade_amd.serve import poisson_requests
+    reqs = poisson_requests(8, 40.0, studies.serving_spec, seed=31)
+    s0 = reqs[0].spec
+    reqs[0].spec = sa.CaseSpec(s0.kind, (128, 128, 128), mach=s0.mach, reynolds=s0.reynolds, steps=s0.steps)
+    for r in reqs:
+        print("request", r.rid, r.spec.kind, r.spec.cells, "steps", r.spec.steps, f"arrival {r.arrival:.3f}s",
+              flush=True)
+        assert r.spec.steps >= 100
+    srv = Server(sa.Batch(8 << 30), max_chunk=8)
+    res = srv.run(reqs)
+    assert len(res) == len(reqs)
+    overlap = sum(1 for r in res for o in res if o is not r and o.admitted < r.completed and r.admitted < o.completed)
+    print("served: rounds", srv.rounds, "overlapping pairs", overlap // 2, flush=True)
+    assert overlap > 0
+    for r in res:
+        b = sa.Batch(1 << 30)
+        cid = b.add(r.request.spec)
+        b.step(r.request.spec.steps)
+        st = b.download(cid)
+        ke, en = b.diag(cid)
+        b.close()
+        assert np.array_equal(r.state, st), r.request.rid
+        assert np.array_equal(r.ke, ke) and np.array_equal(r.enstrophy, en), r.request.rid
+        print("request", r.request.rid, "served == alone", flush=True)

@@ -93,3 +93,22 @@ def test_max_wavespeed_equals_host_value():
         want = np.max(np.sqrt(1.4 * R * T) + np.sqrt(u * u + v * v + w * w))
         got = sa.max_wavespeed(q, gas)
         assert got == want, (trial, got, want)
+
+
+def test_max_wavespeed_passes_nan_through():
+    """A diverged state gives a NaN wave speed (the reference's driver stops on
+    isnan(umax), development/cuda-tgv/main.cc:250-259): one NaN temperature in
+    one interior cell of a block other than the first makes the reduction NaN."""
+    import spade_amd as sa
+    sa.load()
+    g = sa.CartesianGrid((8, 8, 8), (2, 2, 2), ((0.0, 1.0),) * 3, num_exch=2)
+    host = np.zeros(g.host_shape)
+    host[..., 0] = 100.0
+    host[..., 1] = 1.0
+    host[..., 2] = 0.5
+    q = sa.GridArray(g).upload(host)
+    gas = sa.IdealGas(1.4, 100.0)
+    assert np.isfinite(sa.max_wavespeed(q, gas))
+    host[5, 4, 5, 6, 1] = np.nan           # block 5, interior cell (i, j, k) = (4, 3, 2)
+    q.upload(host)
+    assert np.isnan(sa.max_wavespeed(q, gas))

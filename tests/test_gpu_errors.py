@@ -49,7 +49,9 @@ def test_grid_blocks_narrower_than_ghost_layers():
 
 
 @pytest.mark.parametrize("term, words", [
-    (dict(kind=8, order=2), "unknown flux term kind"),                       # hybrid: oracle only
+    (dict(kind=9, order=2), "unknown flux term kind"),
+    (dict(kind=8, order=2), "hybrid dissipative scheme"),                    # diss_kind missing
+    (dict(kind=8, order=6, diss_kind=4), "hybrid central scheme"),
     (dict(kind=1, order=6), "order/2 ghost layers"),
     (dict(kind=1, order=3), "even"),
     (dict(kind=7, law=2, mu=1e-3, prandtl=0.72), "viscous law"),
@@ -90,3 +92,19 @@ def test_wall_face_out_of_range():
     before = q.download()
     _refused(sa, lambda: sa.boundary_fill_wall(q, 6, 1.0), "face")
     assert np.array_equal(q.download(), before)
+
+
+def test_hybrid_outside_its_kernels_unsupported():
+    """hybrid_scheme_t runs only on its compile-time specialised kernels (alone
+    or followed by visc_lr); another composition is a valid request the
+    library does not implement (SPD_ERR_UNSUPPORTED), refused before any launch."""
+    sa = _sa()
+    g, q, rhs = _arrays(sa)
+    hyb = dict(kind=8, order=2, diss_kind=4, split=0, smooth=1, sensor=0, sensor_param=0.5, full=1)
+    pd = dict(kind=3, sensor=0, sensor_param=0.1, eps_p=0.1, eps_T=0.1)
+    with pytest.raises(sa.SpadeError) as e:
+        sa.flux_div(q, rhs, RawFlux([hyb, pd], GAMMA, R), increment=False)
+    assert e.value.code == 3 and "hybrid" in str(e.value), str(e.value)
+    assert np.all(rhs.download() == 7.0)
+    sa.flux_div(q, rhs, RawFlux([hyb, VC], GAMMA, R), increment=False)
+    assert np.all(np.isfinite(rhs.download()))
