@@ -10,7 +10,6 @@ Prints one JSON line (value = Mcell-RHS/s).
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -54,75 +53,64 @@ def tgv_interior(nb, nc, L, p0):
     return out.reshape(nb[0] * nb[1] * nb[2], nc[2], nc[1], nc[0], 5)
 
 
-def cpu_baseline(sample_blocks=4, reps=5):
-    """The compiled reference's own CPU flux_div (oracle/_ref), one core, on a
-    bounded sample of the same workload (TGV, 16^3 blocks, same flux)."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
-    if os.path.exists(exe):
-        try:
-            r = subprocess.run([exe, "--bench", str(sample_blocks), str(reps)], capture_output=True, text=True,
-                               timeout=300, check=True)
-            line = [l for l in r.stdout.splitlines() if l.startswith("{")][-1]
-            d = json.loads(line)
-            return {"value": d["mcell_rhs_per_s"], "unit": "Mcell-RHS/s", "cores": 1, "kind": "reference",
-                    "sample": f"TGV {sample_blocks}^3 blocks of 16^3 ({int(d['cells'])} cells), cent_keep<2>+visc_lr "
-                              f"power law, flux_div basic, median of {reps}",
-                    "cpu": _cpu_model()}
-        except Exception as e:  # pragma: no cover - reported, then the port is timed
-            print(f"reference baseline unavailable: {e}", file=sys.stderr)
+def cpu_baseline(sample_blocks=4, reps=5, max_threads=16, full_blocks=16):
+    """The CPU path timed on this host: the oracle's restatement of the
+    reference's flux_div_basic (pde-algs/flux-div/flux_div_basic.h:17-76 --
+    AoS with the variable fastest, six face evaluations per cell, blocks ->
+    k -> j -> i, FP64, -O3), on a bounded sample of the same workload (TGV,
+    blocks of 16^3, the same flux), median of reps after one untimed call:
+    on one core, and on every core of this host's share (at most max_threads)
+    with OpenMP over blocks, as the reference runs one rank per core over its
+    share of the blocks (grid/partition.h:27-84).  Then one flux_div of the
+    whole full_blocks^3-block grid on every core.  The compiled reference's
+    own one-core rate on the same sample, measured where its sources are
+    (not on this host), is kept in tests/golden/manifest.txt."""
     from oracle import pyoracle as po
-    nb, nc, ng = (sample_blocks,) * 3, (16, 16, 16), 2
-    L = 2.0 * np.pi
-    p0 = 100.0 / 1.4
-    g = po.make_grid(nb, nc, ng, (0, 0, 0), (L, L, L))
-    q = np.zeros(po.shape_of(g))
-    q[:, ng:-ng, ng:-ng, ng:-ng] = tgv_interior(nb, nc, L, p0)
-    po.exchange(g, q)
+    L, p0 = 2.0 * np.pi, 100.0 / 1.4
     fl = po.make_flux(po.make_term(po.TERM_KEEP, order=2),
                       po.make_term(po.TERM_VISC, law=po.LAW_POWER, mu=1 / 1600, T_ref=1.0, power=0.76, prandtl=0.72))
     gas = po.Gas(1.4, p0)
-    po.flux_div(g, gas, fl, q)
-    ts = []
-    for _ in range(reps):
+
+    def state(b):
+        nb, nc, ng = (b,) * 3, (16, 16, 16), 2
+        g = po.make_grid(nb, nc, ng, (0, 0, 0), (L, L, L))
+        q = np.zeros(po.shape_of(g))
+        q[:, ng:-ng, ng:-ng, ng:-ng] = tgv_interior(nb, nc, L, p0)
+        po.exchange(g, q)
+        return g, q, np.prod(nb) * 4096
+
+    def rate(g, q, cells, n):
+        rhs = np.zeros_like(q)
+        po.flux_div(g, gas, fl, q, rhs)
+        ts = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            po.flux_div(g, gas, fl, q, rhs)
+            ts.append(time.perf_counter() - t0)
+        return cells / sorted(ts)[len(ts) // 2] * 1e-6
+
+    nthreads = max(1, min(max_threads, len(os.sched_getaffinity(0))))
+    g, q, cells = state(sample_blocks)
+    po.set_threads(1)
+    one = rate(g, q, cells, reps)
+    used = po.set_threads(nthreads)
+    allc = rate(g, q, cells, reps)
+    out = {"value": round(allc, 4), "unit": "Mcell-RHS/s", "cores": used, "kind": "port",
+           "sample": f"TGV {sample_blocks}^3 blocks of 16^3 ({cells} cells), cent_keep<2> + visc_lr power law, "
+                     f"oracle flux_div (AoS, 6 faces per cell), OpenMP over blocks, median of {reps}",
+           "cpu": _cpu_model(),
+           "one_core": {"value": round(one, 4), "cores": 1, "sample": "same sample, one thread"}}
+    del q
+    if full_blocks:
+        g, q, cells = state(full_blocks)
         t0 = time.perf_counter()
         po.flux_div(g, gas, fl, q)
-        ts.append(time.perf_counter() - t0)
-    cells = np.prod(nb) * 4096
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": cells / sorted(ts)[len(ts) // 2] * 1e-6, "unit": "Mcell-RHS/s", "cores": threads,
-            "kind": "port", "sample": f"TGV {sample_blocks}^3 blocks of 16^3, oracle flux_div, median of {reps}",
-            "cpu": _cpu_model()}
-
-
-def cpu_baseline_all_cores(sample_blocks=4, reps=5, max_procs=16):
-    """The compiled reference's CPU flux_div on every core at once: one
-    single-rank process per core, each on its own sample of blocks, as the
-    reference runs one MPI rank per core over its share of the blocks
-    (partition.h:27-84).  The aggregate is the sum of the processes' rates,
-    all measured while every process runs.  None without the compiled
-    reference."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_driver")
-    if not os.path.exists(exe):
-        return None
-    n = max(1, min(max_procs, len(os.sched_getaffinity(0))))
-    procs = [subprocess.Popen([exe, "--bench", str(sample_blocks), str(reps)], stdout=subprocess.PIPE,
-                              stderr=subprocess.DEVNULL, text=True) for _ in range(n)]
-    rates = []
-    try:
-        for p in procs:
-            out, _ = p.communicate(timeout=300)
-            if p.returncode != 0:
-                raise RuntimeError(f"ref_driver exited with {p.returncode}")
-            rates.append(json.loads([l for l in out.splitlines() if l.startswith("{")][-1])["mcell_rhs_per_s"])
-    except Exception as e:  # pragma: no cover - reported, the one-core figure stands alone
-        for p in procs:
-            if p.poll() is None:
-                p.kill()
-        print(f"all-core reference baseline unavailable: {e}", file=sys.stderr)
-        return None
-    return {"value": round(sum(rates), 4), "unit": "Mcell-RHS/s", "cores": n, "kind": "reference",
-            "sample": f"{n} single-rank processes at once, each TGV {sample_blocks}^3 blocks of 16^3, "
-                      f"median of {reps} each; sum of their rates"}
+        sec = time.perf_counter() - t0
+        out["full_grid"] = {"value": round(cells / sec * 1e-6, 4), "cores": used, "seconds": round(sec, 3),
+                            "sample": f"one flux_div of the whole TGV {full_blocks * 16}^3 grid"}
+        del q
+    po.set_threads(nthreads)
+    return out
 
 
 def rows_measure(sa, g, q, exch, gas, flux, reps=10):
@@ -442,12 +430,7 @@ def main():
     if not args.no_chosen_form:
         line["chosen_form"] = chosen_form(args, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu_baseline:
-        cb = cpu_baseline()
-        if cb["kind"] == "reference":
-            allc = cpu_baseline_all_cores()
-            if allc is not None:
-                cb["all_cores"] = allc
-        line["cpu_baseline"] = cb
+        line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 
 

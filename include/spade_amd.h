@@ -93,8 +93,9 @@ enum {
     SPD_TERM_FIRST,       /* first_order_t (rusanov)     convective.h:14-33 */
     SPD_TERM_VISC,        /* visc_lr                     navier-stokes/viscous.h:13-113 */
     SPD_TERM_HYBRID       /* hybrid_scheme_t: cent_keep<order> blended with weno_t / fweno_t by a face
-                             sensor (navier-stokes/hybrid_scheme.h:14-48, state_sensor.h:21-60); runs on
-                             compile-time specialised kernels only, alone or followed by visc_lr */
+                             sensor (navier-stokes/hybrid_scheme.h:14-48, state_sensor.h:21-60); the
+                             descriptor is defined, but spd_flux_div refuses it (SPD_ERR_UNSUPPORTED):
+                             the library has no device code for it */
 };
 enum { SPD_SENSOR_CONST = 0, SPD_SENSOR_DUCROS = 1 };   /* state_sensor.h:21-60 */
 enum { SPD_SPLIT_RUSANOV = 0, SPD_SPLIT_PHYS = 1 };     /* flux_funcs.h:8-126 */

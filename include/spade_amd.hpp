@@ -149,7 +149,8 @@ inline flux fweno(const ideal_gas_t& gas, bool smooth = true) { auto t = term(SP
 inline flux first_order(const ideal_gas_t& gas) { return flux(term(SPD_TERM_FIRST), gas); }
 
 // hybrid_scheme_t(scheme0, scheme1, blender, full_flux | diss_flux) (navier-stokes/hybrid_scheme.h:14-48):
-// scheme0 = cent_keep<2|4>, scheme1 = weno(...) or fweno(...)
+// scheme0 = cent_keep<2|4>, scheme1 = weno(...) or fweno(...).  flux_div refuses the
+// result (SPD_ERR_UNSUPPORTED): the device library has no code for this scheme.
 enum hybrid_tag { diss_flux = 0, full_flux = 1 };
 inline spd_flux_term hybrid_term(const flux& central, const flux& diss, hybrid_tag tag)
 {

@@ -57,6 +57,7 @@ def _load():
     lib = C.CDLL(path)
     dp = C.POINTER(C.c_double)
     lib.ora_array_size.argtypes = [C.POINTER(Grid)]
+    lib.ora_set_threads.argtypes = [C.c_int]
     lib.ora_flux_div.argtypes = [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Flux), dp, dp, C.c_int]
     lib.ora_flux_div_blocks.argtypes = [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Flux), dp, dp, C.c_int,
                                         C.POINTER(C.c_int), C.c_int]
@@ -120,6 +121,11 @@ def shape_of(g):
 def _ptr(a):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def set_threads(n):
+    """Threads of the oracle's OpenMP loop over blocks; returns the count in use."""
+    return lib().ora_set_threads(int(n))
 
 
 def flux_div(g, gas, flux, prim, rhs=None, increment=False):

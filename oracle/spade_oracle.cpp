@@ -39,6 +39,7 @@
 // for them this file states its own formulation (computational coordinates
 // throughout, gradient scaled by 1/x'(xi) per axis); those paths are pinned by
 // closed-form checks in the tests, not by reference vectors.
+#include <omp.h>
 #include <cmath>
 #include <cstring>
 #include <cstdint>
@@ -746,6 +747,12 @@ int ora_array_size(const ora_grid* g)
 {
     const Geo G(g);
     return 5*G.N[0]*G.N[1]*G.N[2]*G.nblk;
+}
+
+int ora_set_threads(int n)
+{
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
 }
 
 int ora_flux_div(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, const double* prim, double* rhs, int increment)

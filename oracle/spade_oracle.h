@@ -54,6 +54,8 @@ typedef struct { int nterms; ora_term terms[ORA_MAX_TERMS]; } ora_flux;
 typedef struct { int wall_mask; double t_wall; } ora_bc;
 
 int ora_array_size(const ora_grid* g);
+/* threads of the OpenMP loops over blocks (flux_div); returns the count now in use */
+int ora_set_threads(int n);
 int ora_flux_div(const ora_grid* g, const ora_gas* gas, const ora_flux* fl,
                  const double* prim, double* rhs, int increment);
 /* flux_div on the interior cells of the listed blocks only (the others are

@@ -268,7 +268,9 @@ full_flux, diss_flux = 1, 0
 def hybrid_scheme(central, dissipative, sensor, flux_tag=full_flux):
     """hybrid_scheme_t(scheme0, scheme1, blender, tag) (navier-stokes/hybrid_scheme.h:14-48):
     cent_keep<2|4> blended with weno(...) or fweno(...) by the face sensor b
-    (const_sensor or ducros); full_flux = (1 - b) F0 + b F1, diss_flux = F0 + b F1."""
+    (const_sensor or ducros); full_flux = (1 - b) F0 + b F1, diss_flux = F0 + b F1.
+    The descriptor is well formed, but flux_div refuses it (SpadeError, code
+    SPD_ERR_UNSUPPORTED): the device library has no code for this scheme."""
     c, d = central.terms(), dissipative.terms()
     if len(c) != 1 or c[0].fields.get("kind") != _abi.TERM_KEEP:
         raise ValueError("hybrid_scheme: the central scheme must be cent_keep(order, gas)")

@@ -484,38 +484,8 @@ __device__ __forceinline__ void f_visc(const Cell& C, const FaceIdx& f, const sp
     out[4] = -(n[0]*tau[2][0] + n[1]*tau[2][1] + n[2]*tau[2][2]);
 }
 
-// hybrid_scheme_t (navier-stokes/hybrid_scheme.h:30-44): the sensor b of the
-// face (const_sensor_t, or ducros_t from the face gradient, state_sensor.h:
-// 21-60), then the central cent_keep<ORDER> flux F0 and the dissipative
-// weno_t / fweno_t flux F1; full_flux: (1 - b) F0 + b F1, diss_flux: F0 + b F1,
-// each product rounded before the sum, as the reference's f0 *= c0; f1 *= c1;
-// f0 += f1.  Only instantiated in the compile-time specialised kernels, where
-// it is inlined like every other term.
-template <int ORDER, int DISS>
-__device__ inline void f_hybrid(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam,
-                                double out[5])
-{
-    double blend = t.sensor_param;
-    if (t.sensor == SPD_SENSOR_DUCROS)
-    {
-        double gr[3][5]; face_grad(C, f, gr);
-        blend = ducros(gr, t.sensor_param);
-    }
-    double f0[5], f1[5];
-    f_keep(C, f, ORDER, R, gam, f0);
-    if constexpr (DISS == SPD_TERM_FWENO) f_fweno(C, f, t, R, gam, f1);
-    else                                  f_weno(C, f, t, R, gam, f1);
-    double coeff0 = 1.0 - blend;
-    const double coeff1 = blend;
-    if (!t.full) coeff0 = 1.0;
-    for (int n = 0; n < 5; ++n) f0[n] *= coeff0;
-    for (int n = 0; n < 5; ++n) f1[n] *= coeff1;
-    for (int n = 0; n < 5; ++n) out[n] = f0[n] + f1[n];
-}
-
-// The run-time term dispatch of the generic kernel.  SPD_TERM_HYBRID is not
-// among its cases: a hybrid composition is accepted only where a specialised
-// kernel exists for it (flux_div_spec_code), and refused otherwise.
+// The run-time term dispatch of the generic kernel.  SPD_TERM_HYBRID has no
+// device code: validate_flux refuses it before any launch.
 __device__ __forceinline__ void eval_term(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam, double out[5])
 {
     switch (t.kind)
@@ -634,8 +604,6 @@ k_flux_fixup(DevGeom G, const double* __restrict__ prims, double* __restrict__ r
 // k_flux_div_generic needs.  The arithmetic is the same code in the same order
 // as generic_cell, so the two give the same bits.
 constexpr int TERM_KEEP4 = 9;   // cent_keep<4>; SPD_TERM_KEEP stands for cent_keep<2> here
-// hybrid_scheme_t codes: central cent_keep order x dissipative scheme
-constexpr int TERM_HYB2W = 21, TERM_HYB2F = 22, TERM_HYB4W = 23, TERM_HYB4F = 24;
 
 template <int K>
 __device__ inline void eval_static(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam,
@@ -648,10 +616,6 @@ __device__ inline void eval_static(const Cell& C, const FaceIdx& f, const spd_fl
     else if constexpr (K == SPD_TERM_WENO) f_weno(C, f, t, R, gam, out);
     else if constexpr (K == SPD_TERM_FWENO) f_fweno(C, f, t, R, gam, out);
     else if constexpr (K == SPD_TERM_FIRST) f_first(C, f, R, gam, out);
-    else if constexpr (K == TERM_HYB2W) f_hybrid<2, SPD_TERM_WENO>(C, f, t, R, gam, out);
-    else if constexpr (K == TERM_HYB2F) f_hybrid<2, SPD_TERM_FWENO>(C, f, t, R, gam, out);
-    else if constexpr (K == TERM_HYB4W) f_hybrid<4, SPD_TERM_WENO>(C, f, t, R, gam, out);
-    else if constexpr (K == TERM_HYB4F) f_hybrid<4, SPD_TERM_FWENO>(C, f, t, R, gam, out);
     else f_visc(C, f, t, R, gam, out);
 }
 
@@ -773,10 +737,7 @@ k_flux_fixup_spec(DevGeom G, const double* __restrict__ prims, double* __restric
 #define SPD_SPEC_LIST(X) SPD_SPEC_FAST_LIST(X) \
     X(TERM_KEEP4, 0, 0) X(SPD_TERM_TOTANI, 0, 0) X(SPD_TERM_WENO, 0, 0) X(SPD_TERM_FWENO, 0, 0) \
     X(SPD_TERM_FIRST, 0, 0) X(TERM_KEEP4, SPD_TERM_VISC, 0) X(SPD_TERM_TOTANI, SPD_TERM_VISC, 0) \
-    X(SPD_TERM_WENO, SPD_TERM_VISC, 0) X(SPD_TERM_FWENO, SPD_TERM_VISC, 0) X(SPD_TERM_FIRST, SPD_TERM_VISC, 0) \
-    X(TERM_HYB2W, 0, 0) X(TERM_HYB2F, 0, 0) X(TERM_HYB4W, 0, 0) X(TERM_HYB4F, 0, 0) \
-    X(TERM_HYB2W, SPD_TERM_VISC, 0) X(TERM_HYB2F, SPD_TERM_VISC, 0) X(TERM_HYB4W, SPD_TERM_VISC, 0) \
-    X(TERM_HYB4F, SPD_TERM_VISC, 0)
+    X(SPD_TERM_WENO, SPD_TERM_VISC, 0) X(SPD_TERM_FWENO, SPD_TERM_VISC, 0) X(SPD_TERM_FIRST, SPD_TERM_VISC, 0)
 
 bool flux_has_hybrid(const spd_flux_desc& f)
 {
@@ -786,9 +747,9 @@ bool flux_has_hybrid(const spd_flux_desc& f)
 
 int flux_div_spec_code(const spd_flux_desc& f)
 {
-    // SPD_DISABLE_SPEC=1 sends the other compositions to the generic kernel
-    // (for timing comparisons); a hybrid composition has no generic form
-    if (!flux_has_hybrid(f)) if (const char* e = getenv("SPD_DISABLE_SPEC")) if (e[0] == '1') return 0;
+    // SPD_DISABLE_SPEC=1 sends every composition to the generic kernel (for
+    // timing comparisons)
+    if (const char* e = getenv("SPD_DISABLE_SPEC")) if (e[0] == '1') return 0;
     if (f.nterms < 1 || f.nterms > 3) return 0;
     int code = 0;
     for (int m = 0; m < f.nterms; ++m)
@@ -796,13 +757,6 @@ int flux_div_spec_code(const spd_flux_desc& f)
         const spd_flux_term& t = f.terms[m];
         int k = t.kind;
         if (k == SPD_TERM_KEEP) { if (t.order == 4) k = TERM_KEEP4; else if (t.order != 2) return 0; }
-        else if (k == SPD_TERM_HYBRID)
-        {
-            if (t.order != 2 && t.order != 4) return 0;
-            if (t.diss_kind == SPD_TERM_WENO) k = t.order == 2 ? TERM_HYB2W : TERM_HYB4W;
-            else if (t.diss_kind == SPD_TERM_FWENO) k = t.order == 2 ? TERM_HYB2F : TERM_HYB4F;
-            else return 0;
-        }
         else if (k < SPD_TERM_TOTANI || k > SPD_TERM_VISC) return 0;
         code = code*100 + k;
     }
@@ -828,7 +782,6 @@ void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, co
         SPD_SPEC_LIST(SPD_SPEC_LAUNCH)
 #undef SPD_SPEC_LAUNCH
         default:
-            if (flux_has_hybrid(f)) throw ArgError{"hybrid_scheme_t runs only alone or followed by visc_lr"};
             hipLaunchKernelGGL(k_flux_div_generic, dim3((unsigned)nwg), dim3(256), 0, s, G, prims, rhs, f, inc);
     }
     SPD_LAUNCH_CHECK();

@@ -43,5 +43,14 @@ FLUX_CASES = {
 # round-off, not bit for bit, on the GPU (device pow vs glibc pow)
 USES_POW = {"visc_pow", "keep2_visc_pow"}
 
+# compositions the CPU oracle reproduces bit for bit against the reference but
+# the GPU library refuses (SPD_ERR_UNSUPPORTED): its first hybrid_scheme_t
+# kernel build was followed by a GPU memory fault (in an unrelated launch)
+# whose cause was never established, so the library has no device code for
+# it and these cases are skipped there
+GPU_UNSUPPORTED = {k for k in FLUX_CASES if k.startswith("hybrid")}
+GPU_UNSUPPORTED_REASON = ("hybrid_scheme_t is pinned on the CPU oracle only; the device library refuses it "
+                          "(an earlier device build was followed by a memory fault of unestablished cause)")
+
 G1 = dict(nb=(2, 3, 2), nc=(4, 8, 4), ng=2)
 G2 = dict(nb=(2, 2, 2), nc=(8, 8, 8), ng=2)

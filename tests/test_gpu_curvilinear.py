@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import golden_io as gi
-from cases import FLUX_CASES, GAMMA, R
+from cases import FLUX_CASES, GPU_UNSUPPORTED, GPU_UNSUPPORTED_REASON, GAMMA, R
 from gpu_helpers import RawFlux
 from oracle import pyoracle as po
 
@@ -62,6 +62,8 @@ def _oflux(terms):
 
 @pytest.mark.parametrize("name", sorted(FLUX_CASES))
 def test_stretched_flux_div(name):
+    if name in GPU_UNSUPPORTED:
+        pytest.skip(GPU_UNSUPPORTED_REASON)
     sa = _sa()
     dg, og = _grids(sa)
     q = _state(og)
