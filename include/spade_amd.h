@@ -95,7 +95,9 @@ enum {
     SPD_TERM_HYBRID       /* hybrid_scheme_t: cent_keep<order> blended with weno_t / fweno_t by a face
                              sensor (navier-stokes/hybrid_scheme.h:14-48, state_sensor.h:21-60); the
                              descriptor is defined, but spd_flux_div refuses it (SPD_ERR_UNSUPPORTED):
-                             the library has no device code for it */
+                             the library has no device code for it */ ,
+    SPD_TERM_WENO_FDS     /* weno_fds_t<rusanov_fds_t>      convective.h:499-555, flux_funcs.h:54-95 (smooth
+                             selects enable_smooth / disable_smooth) */
 };
 enum { SPD_SENSOR_CONST = 0, SPD_SENSOR_DUCROS = 1 };   /* state_sensor.h:21-60 */
 enum { SPD_SPLIT_RUSANOV = 0, SPD_SPLIT_PHYS = 1 };     /* flux_funcs.h:8-126 */
@@ -109,7 +111,7 @@ typedef struct {
     double sensor_param;             /* const: alpha; Ducros: epsilon */
     double eps_p, eps_T;             /* PDISS */
     int    split;                    /* WENO */
-    int    smooth;                   /* WENO/FWENO: 1 = nonlinear weights (enable_smooth) */
+    int    smooth;                   /* WENO/FWENO/WENO_FDS: 1 = nonlinear weights (enable_smooth) */
     int    law;                      /* VISC */
     double mu, T_ref, power, prandtl;/* VISC: constant mu, or power law mu_ref (T/T_ref)^power */
     int    diss_kind;                /* HYBRID: dissipative scheme, SPD_TERM_WENO or SPD_TERM_FWENO (with

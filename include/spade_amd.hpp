@@ -147,6 +147,11 @@ inline flux weno(const ideal_gas_t& gas, int split, bool smooth = true)
 inline flux fweno(const ideal_gas_t& gas, bool smooth = true) { auto t = term(SPD_TERM_FWENO); t.smooth = smooth; return flux(t, gas); }
 // convective.h:14-33
 inline flux first_order(const ideal_gas_t& gas) { return flux(term(SPD_TERM_FIRST), gas); }
+// weno_fds_t<rusanov_fds_t> (convective.h:499-555, flux_funcs.h:54-95)
+inline flux weno_fds(const ideal_gas_t& gas, bool smooth = true)
+{
+    auto t = term(SPD_TERM_WENO_FDS); t.smooth = smooth; return flux(t, gas);
+}
 
 // hybrid_scheme_t(scheme0, scheme1, blender, full_flux | diss_flux) (navier-stokes/hybrid_scheme.h:14-48):
 // scheme0 = cent_keep<2|4>, scheme1 = weno(...) or fweno(...).  flux_div refuses the

@@ -258,6 +258,11 @@ int main(int argc, char** argv)
         run("fweno", spade::convective::fweno_t<decltype(air)>(air));
         run("fweno_lin", spade::convective::fweno_t<decltype(air), spade::convective::disable_smooth>(air));
         run("first_order", spade::convective::first_order_t<decltype(air)>(air));
+        // face-reconstructed WENO with the Rusanov flux-difference splitting
+        // (convective.h:499-556, flux_funcs.h:54-95)
+        spade::convective::rusanov_fds_t<decltype(air)> rus_fds(air);
+        run("weno_fds", spade::convective::weno_fds_t<decltype(rus_fds)>(rus_fds));
+        run("weno_fds_lin", spade::convective::weno_fds_t<decltype(rus_fds), spade::convective::disable_smooth>(rus_fds));
         run("keep2_pdiss_visc_const", spade::omni::compose(spade::convective::cent_keep<2>(air), pd_ducros, vsc_const));
         // central / WENO blends (navier-stokes/hybrid_scheme.h)
         run("hybrid_keep2_weno_ducros_full",

@@ -22,7 +22,8 @@
 //       gradient four-point rule (omni/infos/info_gradient.h:19-86),
 //       stencil offsets -3,-1,1,3 -> cells i-2,i-1,i,i+1 (omni/geom.h:87-145)
 //   * fluxes: totani_lr, cent_keep<order>, pressure_diss_lr, weno_t, fweno_t,
-//       first_order_t (navier-stokes/convective.h:14-497); rusanov_t,
+//       weno_fds_t, first_order_t (navier-stokes/convective.h:14-556);
+//       rusanov_t, rusanov_fds_t,
 //       phys_flux_t (navier-stokes/flux_funcs.h:8-126); visc_lr
 //       (navier-stokes/viscous.h:39-112); constant_viscosity_t and a power law
 //       with the three getters of power_law_t (navier-stokes/viscous_laws.h:
@@ -513,6 +514,57 @@ void f_fweno(const Ctx& C, const Face& f, const ora_term& t, double out[5])
     }
 }
 
+// weno_fds_t<rusanov_fds_t> (convective.h:499-555, flux_funcs.h:54-95): the
+// primitive states are reconstructed at the face from cells i-2 .. i+1
+// (offsets -3, -1, 1, 3), each component as the reference's prim_t arithmetic
+// rounds it: with nonlinear weights enabled the reference keeps the central
+// candidates ql1 = qr0 = 0.5 q1 + 0.5 q2 ("nothing for now"); with linear
+// weights ql = 2/3 ql1 + 1/3 ql0, qr = 2/3 qr0 + 1/3 qr1.  The Rusanov
+// flux-difference form then takes the face value and face metric:
+// sigma = u_n(face) + c(face) and F = sigma (U(qL) - U(qR)) with U =
+// (rho, rho e_tot, rho u, rho v, rho w), no physical-flux part.
+void f_weno_fds(const Ctx& C, const Face& f, const ora_term& t, double out[5])
+{
+    const Q q0 = cell_at(C.G, C.a, f, -3), q1 = cell_at(C.G, C.a, f, -1);
+    const Q q2 = cell_at(C.G, C.a, f, 1), q3 = cell_at(C.G, C.a, f, 3);
+    double ql[5], qr[5];
+    for (int n = 0; n < 5; ++n)
+    {
+        const double ql0 = -0.5*q0[n] + 1.5*q1[n];
+        const double ql1 =  0.5*q1[n] + 0.5*q2[n];
+        const double qr0 =  0.5*q1[n] + 0.5*q2[n];
+        const double qr1 =  1.5*q2[n] - 0.5*q3[n];
+        ql[n] = ql1;
+        qr[n] = qr0;
+        if (!t.smooth)
+        {
+            ql[n] *= 2.0/3.0;
+            qr[n] *= 2.0/3.0;
+            ql[n] += (1.0/3.0)*ql0;
+            qr[n] += (1.0/3.0)*qr1;
+        }
+    }
+    const Q qf = face_value(C.G, C.a, f);
+    double nv[3]; face_metric(C.G, f, nv);
+    const double R = C.gas.R, gam = C.gas.gamma;
+    const double u_n   = nv[0]*qf.u + nv[1]*qf.v + nv[2]*qf.w;
+    const double rhoL  = ql[0]/(R*ql[1]);
+    const double engyL = 0.5*(ql[2]*ql[2] + ql[3]*ql[3] + ql[4]*ql[4]) + ql[0]/(rhoL*(gam - 1.0));
+    const double rhoR  = qr[0]/(R*qr[1]);
+    const double engyR = 0.5*(qr[2]*qr[2] + qr[3]*qr[3] + qr[4]*qr[4]) + qr[0]/(rhoR*(gam - 1.0));
+    const double sigma = u_n + std::sqrt(gam*R*qf.T);
+    out[0] = sigma*rhoL;
+    out[1] = sigma*rhoL*engyL;
+    out[2] = sigma*rhoL*ql[2];
+    out[3] = sigma*rhoL*ql[3];
+    out[4] = sigma*rhoL*ql[4];
+    out[0] -= sigma*rhoR;
+    out[1] -= sigma*rhoR*engyR;
+    out[2] -= sigma*rhoR*qr[2];
+    out[3] -= sigma*rhoR*qr[3];
+    out[4] -= sigma*rhoR*qr[4];
+}
+
 // first_order_t (convective.h:27-32): fu(L) + fd(R) with rusanov_t
 void f_first(const Ctx& C, const Face& f, double out[5])
 {
@@ -607,6 +659,7 @@ void eval_term(const Ctx& C, const Face& f, const ora_term& t, double out[5])
         case ORA_TERM_FIRST:  f_first(C, f, out); break;
         case ORA_TERM_VISC:   f_visc(C, f, t, out); break;
         case ORA_TERM_HYBRID: f_hybrid(C, f, t, out); break;
+        case ORA_TERM_WENO_FDS: f_weno_fds(C, f, t, out); break;
         default: for (int n = 0; n < 5; ++n) out[n] = 0.0;
     }
 }

@@ -28,7 +28,7 @@ typedef struct
 typedef struct { double gamma, R; } ora_gas;
 
 enum { ORA_TERM_KEEP = 1, ORA_TERM_TOTANI, ORA_TERM_PDISS, ORA_TERM_WENO,
-       ORA_TERM_FWENO, ORA_TERM_FIRST, ORA_TERM_VISC, ORA_TERM_HYBRID };
+       ORA_TERM_FWENO, ORA_TERM_FIRST, ORA_TERM_VISC, ORA_TERM_HYBRID, ORA_TERM_WENO_FDS };
 enum { ORA_SENSOR_CONST = 0, ORA_SENSOR_DUCROS = 1 };
 enum { ORA_SPLIT_RUSANOV = 0, ORA_SPLIT_PHYS = 1 };
 enum { ORA_LAW_CONST = 0, ORA_LAW_POWER = 1 };
@@ -42,7 +42,7 @@ typedef struct
     double sensor_param;        /* const alpha, or Ducros epsilon */
     double eps_p, eps_T;        /* PDISS */
     int split;                  /* WENO */
-    int smooth;                 /* WENO, FWENO: 1 = nonlinear weights */
+    int smooth;                 /* WENO, FWENO, WENO_FDS: 1 = nonlinear weights (enable_smooth) */
     int law;                    /* VISC */
     double mu, T_ref, power, prandtl;
     int diss_kind;              /* HYBRID: dissipative scheme, ORA_TERM_WENO or ORA_TERM_FWENO */

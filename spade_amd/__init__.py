@@ -10,7 +10,8 @@ ABI of libspade_amd.so (include/spade_amd.h):
   grid_array (grid/grid_array.h)                 GridArray
   ideal_gas_t (navier-stokes/gas.h)              IdealGas
   cent_keep<order>, totani_lr, pressure_diss_lr, cent_keep, totani_lr, pressure_diss_lr,
-  weno_t, fweno_t, first_order_t, visc_lr        weno, fweno, first_order, visc_lr
+  weno_t, fweno_t, weno_fds_t, first_order_t,    weno, fweno, weno_fds, first_order,
+  visc_lr                                        visc_lr
   (navier-stokes/convective.h, viscous.h)
   constant_viscosity_t, power_law_t              constant_viscosity, power_law
   const_sensor_t, ducros_t                       const_sensor, ducros
@@ -35,7 +36,7 @@ from ._abi import SpadeError, check, load
 
 __all__ = [
     "Context", "CartesianGrid", "GridArray", "IdealGas", "cent_keep", "totani_lr", "pressure_diss_lr",
-    "weno", "fweno", "first_order", "visc_lr", "constant_viscosity", "power_law", "const_sensor",
+    "weno", "fweno", "weno_fds", "first_order", "visc_lr", "constant_viscosity", "power_law", "const_sensor",
     "ducros", "compose", "flux_div", "create_exchange", "boundary_fill_wall", "Integrator",
     "ssprk3", "ssprk3_opt", "max_wavespeed", "tanh_map", "SpadeError", "diagnostics", "CaseSpec", "Batch",
     "grid_desc", "output_vtk", "output_vtk_host", "binary_write", "binary_read", "PRIM_NAMES", "CONS_NAMES",
@@ -259,6 +260,13 @@ def fweno(gas, smooth=True):
 
 def first_order(gas):
     return _Term(gas, kind=_abi.TERM_FIRST)
+
+
+def weno_fds(gas, smooth=True):
+    """weno_fds_t<rusanov_fds_t> (navier-stokes/convective.h:499-555,
+    flux_funcs.h:54-95).  With smooth=True (enable_smooth) the reference's
+    face states are both the central value, so its flux is identically zero."""
+    return _Term(gas, kind=_abi.TERM_WENO_FDS, smooth=int(bool(smooth)))
 
 
 # hybrid_scheme_t flux tags (navier-stokes/hybrid_scheme.h:7-12)

@@ -11,7 +11,8 @@ LIB_PATH = os.path.join(HERE, "libspade_amd.so")
 
 SPD_OK, SPD_ERR_ARG, SPD_ERR_HIP, SPD_ERR_UNSUPPORTED, SPD_ERR_MEMORY, SPD_ERR_IO = range(6)
 MAP_IDENTITY, MAP_TANH = 0, 1
-TERM_KEEP, TERM_TOTANI, TERM_PDISS, TERM_WENO, TERM_FWENO, TERM_FIRST, TERM_VISC, TERM_HYBRID = range(1, 9)
+TERM_KEEP, TERM_TOTANI, TERM_PDISS, TERM_WENO, TERM_FWENO, TERM_FIRST, TERM_VISC, TERM_HYBRID, TERM_WENO_FDS = \
+    range(1, 10)
 SENSOR_CONST, SENSOR_DUCROS = 0, 1
 SPLIT_RUSANOV, SPLIT_PHYS = 0, 1
 LAW_CONST, LAW_POWER = 0, 1

@@ -43,6 +43,9 @@ void validate_flux(const spd_flux_desc* f, int ng)
                 require(t.order/2 <= ng, "cent_keep order needs order/2 ghost layers");
                 require(t.order <= 4, "cent_keep orders above 4 are not implemented");
                 break;
+            case SPD_TERM_WENO_FDS:
+                require(ng >= 2, "WENO stencils need 2 ghost layers");
+                break;
             case SPD_TERM_WENO: case SPD_TERM_FWENO:
                 require(ng >= 2, "WENO stencils need 2 ghost layers");
                 require(t.kind == SPD_TERM_FWENO || t.split == SPD_SPLIT_RUSANOV || t.split == SPD_SPLIT_PHYS,

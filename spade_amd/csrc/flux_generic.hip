@@ -343,6 +343,57 @@ __device__ __forceinline__ void f_weno(const Cell& C, const FaceIdx& f, const sp
     }
 }
 
+// weno_fds_t<rusanov_fds_t> (convective.h:499-555, flux_funcs.h:54-95): the
+// face states are reconstructed from cells i-2 .. i+1 component by component
+// (enable_smooth keeps the central candidates, disable_smooth blends them
+// with the one-sided ones at 2/3 : 1/3); the Rusanov flux difference then
+// takes the face value and face metric: sigma = u_n + c at the face and
+// F = sigma (U(qL) - U(qR)), U = (rho, rho e_tot, rho u, rho v, rho w).
+__device__ __forceinline__ void f_weno_fds(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R,
+                                           double gam, double out[5])
+{
+    const Prim a0 = cell_at(C, f, -3), a1 = cell_at(C, f, -1), a2 = cell_at(C, f, 1), a3 = cell_at(C, f, 3);
+    const double q0[5] = {a0.p, a0.T, a0.u, a0.v, a0.w}, q1[5] = {a1.p, a1.T, a1.u, a1.v, a1.w};
+    const double q2[5] = {a2.p, a2.T, a2.u, a2.v, a2.w}, q3[5] = {a3.p, a3.T, a3.u, a3.v, a3.w};
+    double ql[5], qr[5];
+    for (int n = 0; n < 5; ++n)
+    {
+        const double ql0 = -0.5*q0[n] + 1.5*q1[n];
+        const double ql1 =  0.5*q1[n] + 0.5*q2[n];
+        const double qr0 =  0.5*q1[n] + 0.5*q2[n];
+        const double qr1 =  1.5*q2[n] - 0.5*q3[n];
+        ql[n] = ql1;
+        qr[n] = qr0;
+        if (!t.smooth)
+        {
+            ql[n] *= 2.0/3.0;
+            qr[n] *= 2.0/3.0;
+            ql[n] += (1.0/3.0)*ql0;
+            qr[n] += (1.0/3.0)*qr1;
+        }
+    }
+    // face value (omni/infos/info_value.h:151-163): (q(-1) + q(1)) * 0.5
+    double qf[5];
+    for (int n = 0; n < 5; ++n) { qf[n] = q1[n]; qf[n] += q2[n]; qf[n] *= 0.5; }
+    double nv[3]; face_metric(C, f, nv);
+    const double u_n   = nv[0]*qf[2] + nv[1]*qf[3] + nv[2]*qf[4];
+    const double rhoL  = ql[0]/(R*ql[1]);
+    const double engyL = 0.5*(ql[2]*ql[2] + ql[3]*ql[3] + ql[4]*ql[4]) + ql[0]/(rhoL*(gam - 1.0));
+    const double rhoR  = qr[0]/(R*qr[1]);
+    const double engyR = 0.5*(qr[2]*qr[2] + qr[3]*qr[3] + qr[4]*qr[4]) + qr[0]/(rhoR*(gam - 1.0));
+    const double sigma = u_n + sqrt(gam*R*qf[1]);
+    out[0] = sigma*rhoL;
+    out[1] = sigma*rhoL*engyL;
+    out[2] = sigma*rhoL*ql[2];
+    out[3] = sigma*rhoL*ql[3];
+    out[4] = sigma*rhoL*ql[4];
+    out[0] -= sigma*rhoR;
+    out[1] -= sigma*rhoR*engyR;
+    out[2] -= sigma*rhoR*qr[2];
+    out[3] -= sigma*rhoR*qr[3];
+    out[4] -= sigma*rhoR*qr[4];
+}
+
 // fweno_t (convective.h:355-496)
 __device__ __forceinline__ void f_fweno(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam, double out[5])
 {
@@ -497,6 +548,7 @@ __device__ __forceinline__ void eval_term(const Cell& C, const FaceIdx& f, const
         case SPD_TERM_FWENO:  f_fweno(C, f, t, R, gam, out); break;
         case SPD_TERM_FIRST:  f_first(C, f, R, gam, out); break;
         case SPD_TERM_VISC:   f_visc(C, f, t, R, gam, out); break;
+        case SPD_TERM_WENO_FDS: f_weno_fds(C, f, t, R, gam, out); break;
         default: for (int n = 0; n < 5; ++n) out[n] = 0.0;
     }
 }
@@ -603,7 +655,7 @@ k_flux_fixup(DevGeom G, const double* __restrict__ prims, double* __restrict__ r
 // registers instead of the per-lane stack that the run-time dispatch of
 // k_flux_div_generic needs.  The arithmetic is the same code in the same order
 // as generic_cell, so the two give the same bits.
-constexpr int TERM_KEEP4 = 9;   // cent_keep<4>; SPD_TERM_KEEP stands for cent_keep<2> here
+constexpr int TERM_KEEP4 = 19;  // cent_keep<4>; SPD_TERM_KEEP stands for cent_keep<2> here
 
 template <int K>
 __device__ inline void eval_static(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam,
@@ -616,6 +668,7 @@ __device__ inline void eval_static(const Cell& C, const FaceIdx& f, const spd_fl
     else if constexpr (K == SPD_TERM_WENO) f_weno(C, f, t, R, gam, out);
     else if constexpr (K == SPD_TERM_FWENO) f_fweno(C, f, t, R, gam, out);
     else if constexpr (K == SPD_TERM_FIRST) f_first(C, f, R, gam, out);
+    else if constexpr (K == SPD_TERM_WENO_FDS) f_weno_fds(C, f, t, R, gam, out);
     else f_visc(C, f, t, R, gam, out);
 }
 
@@ -737,7 +790,8 @@ k_flux_fixup_spec(DevGeom G, const double* __restrict__ prims, double* __restric
 #define SPD_SPEC_LIST(X) SPD_SPEC_FAST_LIST(X) \
     X(TERM_KEEP4, 0, 0) X(SPD_TERM_TOTANI, 0, 0) X(SPD_TERM_WENO, 0, 0) X(SPD_TERM_FWENO, 0, 0) \
     X(SPD_TERM_FIRST, 0, 0) X(TERM_KEEP4, SPD_TERM_VISC, 0) X(SPD_TERM_TOTANI, SPD_TERM_VISC, 0) \
-    X(SPD_TERM_WENO, SPD_TERM_VISC, 0) X(SPD_TERM_FWENO, SPD_TERM_VISC, 0) X(SPD_TERM_FIRST, SPD_TERM_VISC, 0)
+    X(SPD_TERM_WENO, SPD_TERM_VISC, 0) X(SPD_TERM_FWENO, SPD_TERM_VISC, 0) X(SPD_TERM_FIRST, SPD_TERM_VISC, 0) \
+    X(SPD_TERM_WENO_FDS, 0, 0) X(SPD_TERM_WENO_FDS, SPD_TERM_VISC, 0)
 
 bool flux_has_hybrid(const spd_flux_desc& f)
 {
@@ -757,7 +811,7 @@ int flux_div_spec_code(const spd_flux_desc& f)
         const spd_flux_term& t = f.terms[m];
         int k = t.kind;
         if (k == SPD_TERM_KEEP) { if (t.order == 4) k = TERM_KEEP4; else if (t.order != 2) return 0; }
-        else if (k < SPD_TERM_TOTANI || k > SPD_TERM_VISC) return 0;
+        else if ((k < SPD_TERM_TOTANI || k > SPD_TERM_VISC) && k != SPD_TERM_WENO_FDS) return 0;
         code = code*100 + k;
     }
     for (int m = f.nterms; m < 3; ++m) code *= 100;

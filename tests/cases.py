@@ -29,6 +29,9 @@ FLUX_CASES = {
     "fweno": [dict(kind=5, smooth=1)],
     "fweno_lin": [dict(kind=5, smooth=0)],
     "first_order": [dict(kind=6)],
+    # weno_fds_t<rusanov_fds_t>: face-reconstructed states, Rusanov flux difference
+    "weno_fds": [dict(kind=9, smooth=1)],
+    "weno_fds_lin": [dict(kind=9, smooth=0)],
     "keep2_pdiss_visc_const": [K2, PDD, VC],
     # hybrid_scheme_t: cent_keep<order> blended with WENO by a face sensor
     "hybrid_keep2_weno_ducros_full": [dict(kind=8, order=2, sensor=1, sensor_param=1e-3, split=0, smooth=1,

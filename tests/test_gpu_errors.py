@@ -49,7 +49,7 @@ def test_grid_blocks_narrower_than_ghost_layers():
 
 
 @pytest.mark.parametrize("term, words", [
-    (dict(kind=9, order=2), "unknown flux term kind"),
+    (dict(kind=10, order=2), "unknown flux term kind"),
     (dict(kind=8, order=2), "hybrid dissipative scheme"),                    # diss_kind missing
     (dict(kind=8, order=6, diss_kind=4), "hybrid central scheme"),
     (dict(kind=1, order=6), "order/2 ghost layers"),
@@ -73,7 +73,8 @@ def test_flux_descriptor_refused(term, words):
 def test_weno_needs_two_ghost_layers():
     sa = _sa()
     g, q, rhs = _arrays(sa, ng=1)
-    _refused(sa, lambda: sa.flux_div(q, rhs, RawFlux([dict(kind=4, split=0, smooth=1)], GAMMA, R)), "2 ghost layers")
+    for t in (dict(kind=4, split=0, smooth=1), dict(kind=5, smooth=1), dict(kind=9, smooth=0)):
+        _refused(sa, lambda: sa.flux_div(q, rhs, RawFlux([t], GAMMA, R)), "2 ghost layers")
 
 
 def test_arrays_of_different_grids_or_the_same_array():

@@ -3,7 +3,8 @@ same arithmetic in the same order as the generic per-cell kernel, so on the
 same input the two give the same residual bit for bit.  Compositions: every
 non-empty subsequence of cent_keep<2>, pressure_diss_lr and visc_lr (in that
 order), and cent_keep<4>, totani_lr, weno_t (Rusanov and physical splitting,
-smooth and linear weights), fweno_t and first_order_t, each alone and followed
+smooth and linear weights), fweno_t, first_order_t and weno_fds_t (linear
+weights), each alone and followed
 by visc_lr.  Checked on the stretched wall-bounded grid (where the LDS-staged kernel
 does not apply) and on an identity grid with 8^3 blocks (nor there), in
 overwrite and increment modes; SPD_DISABLE_SPEC=1 selects the generic kernel.
@@ -32,7 +33,8 @@ VISC = [dict(kind=7, law=0, mu=2e-3, prandtl=0.72),
 OTHERS = [("keep4", dict(kind=1, order=4)), ("totani", dict(kind=2)),
           ("weno_rus", dict(kind=4, split=0, smooth=1)), ("weno_rus_lin", dict(kind=4, split=0, smooth=0)),
           ("weno_phys", dict(kind=4, split=1, smooth=1)), ("fweno", dict(kind=5, smooth=1)),
-          ("fweno_lin", dict(kind=5, smooth=0)), ("first_order", dict(kind=6))]
+          ("fweno_lin", dict(kind=5, smooth=0)), ("first_order", dict(kind=6)),
+          ("weno_fds_lin", dict(kind=9, smooth=0))]
 
 
 def _compositions():

@@ -17,6 +17,20 @@
   fields that couple the axes.  Dropping the metric factor of the tangential
   gradient components in a copy of the oracle gives O(1) errors and order ~0
   here, while freestream, conservation and the linear field still pass.
+* weno_fds_t<rusanov_fds_t> (convective.h:499-555, flux_funcs.h:54-95) is a
+  pure flux difference sigma (U(qL) - U(qR)): with nonlinear weights enabled
+  its two face states are the same central value (the reference's nonlinear
+  branch is empty), so the residual is exactly zero; with linear weights the
+  left and right three-point reconstructions agree on every field quadratic in
+  the cell index, so the residual vanishes to round-off there and not for a
+  cubic (a wrong stencil weight breaks the cancellation).  On a grid stretched
+  in y the y-faces have unit area, so a field varying along y only gives the
+  identity grid's residual divided by the cell's y' (the Jacobian); an x-face
+  has area y', which enters the Rusanov speed sigma = n.u + c only through
+  n.u (the reference's formula: c is not area-weighted), so a field at rest
+  varying along x only also gives the identity residual over y'.  (The face
+  normal itself is the one every scheme uses, pinned by the convergence
+  orders below.)
 * Convergence of every convective flux on the same stretched grid towards
   the Euler divergence derived symbolically, at its formal order (cent_keep<2>
   and totani 2, cent_keep<4> 4, WENO / FWENO with linear weights 3,
@@ -47,6 +61,7 @@ ALL = {
                                 split=0, smooth=1, diss_kind=po.TERM_WENO, full=1)],
     "hybrid_fweno_const_diss": [dict(kind=po.TERM_HYBRID, order=4, sensor=po.SENSOR_CONST, sensor_param=0.3,
                                      smooth=1, diss_kind=po.TERM_FWENO, full=0)],
+    "weno_fds_lin": [dict(kind=po.TERM_WENO_FDS, smooth=0)],
 }
 TANH_Y = (po.MAP_TANH, -1.0, 1.0, 1.5)
 
@@ -337,3 +352,62 @@ def test_convective_order_on_stretched_grid(name):
     order = np.log2(errs[:-1] / errs[1:])
     print(name, "stretched errors", errs[-1], "orders", order[-1])
     assert np.all(order[-1] > p - 0.2), order
+
+
+FDS = dict(kind=po.TERM_WENO_FDS, smooth=0)
+
+
+def _index_field(g, axis, degree, at_rest=False):
+    """A state varying only along `axis` as a polynomial of the cell index
+    (ghosts included), with positive pressure and temperature; velocity zero
+    when at_rest."""
+    q = np.zeros(po.shape_of(g))
+    nb, nc = tuple(g.nb), tuple(g.nc)
+    ng = 2
+    for lb in range(q.shape[0]):
+        b = (lb % nb[0], (lb // nb[0]) % nb[1], lb // (nb[0] * nb[1]))
+        s = (b[axis] * nc[axis] + np.arange(-ng, nc[axis] + ng)) / (nb[axis] * nc[axis])
+        shape = [1, 1, 1]
+        shape[2 - axis] = len(s)
+        x = (s - 0.5).reshape(shape)
+        poly = x ** degree
+        q[lb, ..., 0] = 101325.0 * (1 + 0.02 * poly)
+        q[lb, ..., 1] = 300.0 * (1 - 0.03 * poly)
+        q[lb, ..., 2] = 30.0 + 10.0 * poly
+        q[lb, ..., 3] = -12.0 + 7.0 * poly
+        q[lb, ..., 4] = 7.0 - 5.0 * poly
+        if at_rest:
+            q[lb, ..., 2:] = 0.0
+    return q
+
+
+def test_weno_fds_smooth_is_zero():
+    g = _grid(maps=(None, TANH_Y, None))
+    q = _rough_state(g, 3)
+    po.exchange(g, q)
+    rhs = po.flux_div(g, GAS, _flux([dict(kind=po.TERM_WENO_FDS, smooth=1)]), q)[:, 2:-2, 2:-2, 2:-2]
+    assert np.all(rhs == 0.0)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_weno_fds_linear_weights_cancel_on_quadratics(axis):
+    g = _grid(periodic=(0, 0, 0))
+    quad = po.flux_div(g, GAS, _flux([FDS]), _index_field(g, axis, 2))[:, 2:-2, 2:-2, 2:-2]
+    cubic = po.flux_div(g, GAS, _flux([FDS]), _index_field(g, axis, 3))[:, 2:-2, 2:-2, 2:-2]
+    for v in range(5):
+        assert np.max(np.abs(quad[..., v])) <= 1e-5 * np.max(np.abs(cubic[..., v])), v
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_weno_fds_face_areas_on_stretched_grid(axis):
+    gi = _grid(periodic=(0, 0, 0))
+    gs = _grid(periodic=(0, 0, 0), maps=(None, TANH_Y, None))
+    q = _index_field(gi, axis, 3, at_rest=(axis == 0))
+    ri = po.flux_div(gi, GAS, _flux([FDS]), q)[:, 2:-2, 2:-2, 2:-2]
+    rs = po.flux_div(gs, GAS, _flux([FDS]), q)[:, 2:-2, 2:-2, 2:-2]
+    rs = rs * _volume(gs)[..., None]            # y' of each cell (x' = z' = 1)
+    vs = [0, 1] if axis == 0 else range(5)       # momentum is zero at rest
+    scale = np.max(np.abs(ri), axis=(0, 1, 2, 3))
+    err = np.max(np.abs(rs - ri), axis=(0, 1, 2, 3))
+    for v in vs:
+        assert scale[v] > 0.0 and err[v] <= 1e-12 * scale[v], (v, err[v], scale[v])
