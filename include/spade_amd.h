@@ -155,6 +155,40 @@ int spd_exchange(spd_array_t arr);
  * (grid/boundary_fill.h:32-133).  face = 2*dir + (0: min, 1: max). */
 int spd_boundary_fill_wall(spd_array_t arr, int face, double t_wall);
 
+/* Faces of a boundary_fill call: bit (2*dir + pm), pm 0 = min, 1 = max; the
+ * faces are filled in face order xmin, xmax, ymin, ymax, zmin, zmax (later
+ * faces read the ghosts earlier ones wrote at edges and corners), on the
+ * blocks whose face lies on the domain boundary, every ghost layer beyond the
+ * face over the whole stored tangential range (grid/boundary_fill.h:32-60;
+ * the boundary::identifier_t of boundary_fill.h:9-15). */
+enum { SPD_XMIN = 1, SPD_XMAX = 2, SPD_YMIN = 4, SPD_YMAX = 8, SPD_ZMIN = 16, SPD_ZMAX = 32 };
+
+/* boundary_fill(arr, faces, boundary::extrapolate<order>) (boundary_fill.h:
+ * 17-26, 63-98): Lagrange extrapolation along the face normal through the
+ * order + 1 interior cells nearest a min face / farthest cells ending at n - 1
+ * of a max face.  Needs order + 1 <= cells per block along each filled axis. */
+int spd_boundary_fill_extrap(spd_array_t arr, int face_mask, int order);
+
+/* boundary_fill(arr, faces, kernel) with a user kernel (boundary_fill.h:
+ * 99-127), kern(q_image, q_ghost, x, idir) or kern(q_image, idir).  An
+ * arbitrary callable cannot cross a C ABI, so the kernel is given as data:
+ * per primitive variable v (p, T, u, v, w)
+ *     ghost[v] = dom[v] * q_image[v] + fill[v] * q_ghost[v] + add[v]
+ * where q_image is the mirror cell (-1 - i at a min face, 2n - (i + 1) at a
+ * max face) and q_ghost the ghost's current value; the velocity component
+ * normal to the face takes dom_normal / fill_normal / add_normal instead.  A
+ * term with a zero coefficient is left out; the others are rounded in the
+ * order written (so e.g. the no-slip isothermal wall, the slip wall, a fixed
+ * state or a zero-gradient copy give the bits of the same kernel written as a
+ * reference lambda).  Kernels that depend on the ghost's coordinates x have no
+ * descriptor. */
+typedef struct
+{
+    double dom[5], fill[5], add[5];
+    double dom_normal, fill_normal, add_normal;
+} spd_fill_kernel;
+int spd_boundary_fill(spd_array_t arr, int face_mask, const spd_fill_kernel* kern);
+
 /* ---- time integration: integrator_t(axis, scheme, data, rhs, bc, trans).advance()
  *      (time-integration/integrator.h:18-59) with rhs = flux_div(overwrite),
  *      bc = exchange + walls, trans = prim<->cons state transform. ---- */

@@ -211,6 +211,22 @@ inline exchange_handle create_exchange(const grid_array&) { return exchange_hand
 // boundary_fill with the isothermal no-slip wall mirror (grid/boundary_fill.h:32-133)
 inline void boundary_fill_wall(grid_array& a, int face, double t_wall) { check(spd_boundary_fill_wall(a.handle(), face, t_wall)); }
 
+// algs::boundary_fill(arr, boundaries, kern) (grid/boundary_fill.h:32-133):
+// boundaries = boundary::xmin | boundary::ymax | ...; kern = boundary::
+// extrapolate<order> or a fill_kernel (a user kernel given as data)
+namespace boundary
+{
+constexpr int xmin = SPD_XMIN, xmax = SPD_XMAX, ymin = SPD_YMIN, ymax = SPD_YMAX, zmin = SPD_ZMIN, zmax = SPD_ZMAX;
+template <int order> struct extrap_t { static constexpr int value = order; };
+template <int order> constexpr extrap_t<order> extrapolate{};
+}
+using fill_kernel = spd_fill_kernel;
+template <int order> inline void boundary_fill(grid_array& a, int faces, boundary::extrap_t<order>)
+{
+    check(spd_boundary_fill_extrap(a.handle(), faces, order));
+}
+inline void boundary_fill(grid_array& a, int faces, const fill_kernel& k) { check(spd_boundary_fill(a.handle(), faces, &k)); }
+
 enum scheme { ssprk3 = SPD_SCHEME_SSPRK3, ssprk3_opt = SPD_SCHEME_SSPRK3_OPT };
 
 // integrator_t(axis, scheme, data, rhs = flux_div, bc = exchange + walls,

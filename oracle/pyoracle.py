@@ -47,6 +47,11 @@ class Flux(C.Structure):
     _fields_ = [("nterms", C.c_int), ("terms", Term * MAX_TERMS)]
 
 
+class FillKernel(C.Structure):
+    _fields_ = [("dom", C.c_double * 5), ("fill", C.c_double * 5), ("add", C.c_double * 5),
+                ("dom_normal", C.c_double), ("fill_normal", C.c_double), ("add_normal", C.c_double)]
+
+
 class BC(C.Structure):
     _fields_ = [("wall_mask", C.c_int), ("t_wall", C.c_double)]
 
@@ -64,6 +69,8 @@ def _load():
                                         C.POINTER(C.c_int), C.c_int]
     lib.ora_exchange.argtypes = [C.POINTER(Grid), dp]
     lib.ora_boundary_fill_wall.argtypes = [C.POINTER(Grid), dp, C.c_int, C.c_double]
+    lib.ora_boundary_fill_extrap.argtypes = [C.POINTER(Grid), dp, C.c_int, C.c_int]
+    lib.ora_boundary_fill_affine.argtypes = [C.POINTER(Grid), dp, C.c_int, C.POINTER(FillKernel)]
     lib.ora_prim_to_cons.argtypes = [C.POINTER(Gas), dp, dp, C.c_long]
     lib.ora_cons_to_prim.argtypes = [C.POINTER(Gas), dp, dp, C.c_long]
     lib.ora_ssprk3_step.argtypes = [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Flux), C.POINTER(BC),
@@ -157,6 +164,31 @@ def exchange(g, a):
 
 def boundary_fill_wall(g, a, face, t_wall):
     lib().ora_boundary_fill_wall(C.byref(g), _ptr(a), face, t_wall)
+    return a
+
+
+def boundary_fill_extrap(g, a, face_mask, order):
+    if lib().ora_boundary_fill_extrap(C.byref(g), _ptr(a), face_mask, order) != 0:
+        raise RuntimeError("ora_boundary_fill_extrap failed")
+    return a
+
+
+def fill_kernel(dom, fill=(0.0,) * 5, add=(0.0,) * 5, dom_normal=None, fill_normal=None, add_normal=None):
+    """Affine boundary kernel: ghost[v] = dom[v] image + fill[v] ghost + add[v];
+    the face-normal velocity takes dom_normal / fill_normal / add_normal,
+    which default to the u entries (index 2)."""
+    k = FillKernel()
+    for v in range(5):
+        k.dom[v], k.fill[v], k.add[v] = dom[v], fill[v], add[v]
+    k.dom_normal = dom[2] if dom_normal is None else dom_normal
+    k.fill_normal = fill[2] if fill_normal is None else fill_normal
+    k.add_normal = add[2] if add_normal is None else add_normal
+    return k
+
+
+def boundary_fill_affine(g, a, face_mask, kern):
+    if lib().ora_boundary_fill_affine(C.byref(g), _ptr(a), face_mask, C.byref(kern)) != 0:
+        raise RuntimeError("ora_boundary_fill_affine failed")
     return a
 
 

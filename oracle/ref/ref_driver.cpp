@@ -355,6 +355,69 @@ int main(int argc, char** argv)
     }
 
     //------------------------------------------------------------------
+    // Grid G4: same layout, no periodic axis.  After the exchange (which
+    // leaves every domain face alone), boundary_fill with Lagrange
+    // extrapolation on all six faces (face order xmin .. zmax matters at
+    // edges and corners), extrapolation of order 3 on two faces, and a
+    // four-argument user kernel (slip wall: normal velocity mirrored,
+    // temperature isothermal, pressure blended with the ghost's own value).
+    //------------------------------------------------------------------
+    {
+        spade::ctrs::array<int, 3> num_blocks(2, 3, 2);
+        spade::ctrs::array<int, 3> cells_in_block(4, 8, 4);
+        spade::ctrs::array<int, 3> nexch(2, 2, 2);
+        spade::bound_box_t<real_t, 3> bounds;
+        for (int d = 0; d < 3; ++d) { bounds.min(d) = 0.0; bounds.max(d) = twopi; }
+        spade::grid::cartesian_blocks_t blocks(num_blocks, bounds);
+        fix_boundary_flags(blocks);
+        spade::grid::cartesian_grid_t grid(cells_in_block, blocks, coords, group);
+        prim_t fill1 = 0.0;
+        spade::grid::grid_array prim(grid, fill1, nexch, spade::device::cpu);
+        auto init = [&]()
+        {
+            auto img = prim.image();
+            for (int lb = 0; lb < (int)grid.get_num_local_blocks(); ++lb)
+            for (int k = -2; k < 6; ++k) for (int j = -2; j < 10; ++j) for (int i = -2; i < 6; ++i)
+            {
+                spade::grid::cell_idx_t idx(i, j, k, lb);
+                prim_t q;
+                q.p() = p0 + 0.01*(lb + 1) + 0.001*(i + 2) + 0.0001*(j + 2) + 0.00001*(k + 2);
+                q.T() = T0 + 0.02*rough(i, j, k + 16*lb, 1);
+                q.u() = 0.3*rough(i, j, k + 16*lb, 2);
+                q.v() = 0.3*rough(i, j, k + 16*lb, 3);
+                q.w() = 0.3*rough(i, j, k + 16*lb, 4);
+                img.set_elem(idx, q);
+            }
+            auto handle = spade::grid::create_exchange(prim, group, spade::ctrs::array<bool,3>(false, false, false));
+            handle.exchange(prim);
+        };
+        const spade::boundary::identifier_t all_faces(true, true, true, true, true, true);
+        init();
+        dump("g4_prim_exch_ghost", prim, true);
+        spade::algs::boundary_fill(prim, all_faces, spade::boundary::extrapolate<2>);
+        dump("g4_prim_extrap2_ghost", prim, true);
+        init();
+        spade::algs::boundary_fill(prim, all_faces, spade::boundary::extrapolate<0>);
+        dump("g4_prim_extrap0_ghost", prim, true);
+        init();
+        const spade::boundary::identifier_t xmin_ymax(true, false, false, true, false, false);
+        spade::algs::boundary_fill(prim, xmin_ymax, spade::boundary::extrapolate<3>);
+        dump("g4_prim_extrap3_xmin_ymax_ghost", prim, true);
+        init();
+        const real_t t_wall = 0.9;
+        auto slip = [=](const prim_t& q_dom, const prim_t& q_fill, const spade::coords::point_t<real_t>& x, const int& idir) -> prim_t
+        {
+            prim_t g = q_dom;
+            g.p() = 0.25*q_fill.p() + q_dom.p();
+            g.T() = 2.0*t_wall - q_dom.T();
+            g.u(idir) = -q_dom.u(idir);
+            return g;
+        };
+        spade::algs::boundary_fill(prim, all_faces, slip);
+        dump("g4_prim_slip_ghost", prim, true);
+    }
+
+    //------------------------------------------------------------------
     // Grid G2: TGV 16^3 as 2x2x2 blocks of 8^3, ng=2: cent_keep<2> +
     // visc_lr(power law), SSP-RK3 (transform-aware) 1 and 10 steps,
     // and the low-storage ssprk3_opt for 3 steps.

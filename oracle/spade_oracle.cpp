@@ -794,6 +794,38 @@ void exchange(const Geo& G, double* a)
 
 } // namespace
 
+// boundary_fill over the faces in face_mask, in face order xmin, xmax, ymin,
+// ymax, zmin, zmax (grid/boundary_fill.h:32-133): on every block whose face
+// lies on the domain boundary, every ghost cell beyond that face (normal index
+// outside [0, n), tangential indices over the whole stored range) gets
+// ghost(fill index, image cell value, current ghost value).
+template <typename F> void fill_faces(const Geo& G, double* a, int face_mask, F&& ghost)
+{
+    const int ng = G.g.ng;
+    for (int face = 0; face < 6; ++face)
+    {
+        if (!(face_mask & (1 << face))) continue;
+        const int idir = face/2, pm = face % 2;
+        for (int lb = 0; lb < G.nblk; ++lb)
+        {
+            const int bp = G.bpos[3*lb + idir];
+            if (pm == 0 && bp != 0) continue;
+            if (pm == 1 && bp != G.g.nb[idir] - 1) continue;
+            int ll[3], ur[3];
+            for (int d = 0; d < 3; ++d) { ll[d] = -ng; ur[d] = G.g.nc[d] + ng; }
+            if (pm == 0) ur[idir] = 0;
+            if (pm == 1) ll[idir] = G.g.nc[idir];
+            for (int k = ll[2]; k < ur[2]; ++k)
+            for (int j = ll[1]; j < ur[1]; ++j)
+            for (int i = ll[0]; i < ur[0]; ++i)
+            {
+                const int fi[3] = {i, j, k};
+                ghost(lb, idir, pm, fi, a + G.cell(i, j, k, lb));
+            }
+        }
+    }
+}
+
 extern "C" {
 
 int ora_array_size(const ora_grid* g)
@@ -875,6 +907,72 @@ int ora_boundary_fill_wall(const ora_grid* g, double* a, int face, double t_wall
             for (int v = 0; v < 5; ++v) dd[v] = q[v];
         }
     }
+    return 0;
+}
+
+// extrap_t<order> (boundary_fill.h:17-26, 63-98): Lagrange extrapolation
+// through the order + 1 cells i0 .. i0 + order along the face normal (i0 = 0
+// at a min face, n - order - 1 at a max face), basis weights built as
+// l_j = l_j (x - x_i) / (x_j - x_i) over ascending i != j on integer cell
+// indices, the ghost accumulated as g = g + l_j q_j from g = 0.
+int ora_boundary_fill_extrap(const ora_grid* g, double* a, int face_mask, int order)
+{
+    if (!g || !a || order < 0) return -1;
+    const Geo G(g);
+    for (int d = 0; d < 3; ++d) if (order + 1 > G.g.nc[d] && (face_mask & (3 << (2*d)))) return -1;
+    fill_faces(G, a, face_mask, [&](int lb, int idir, int pm, const int fi[3], double* dst)
+    {
+        const int i0 = pm*(G.g.nc[idir] - order - 1);
+        double gv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int j = 0; j <= order; ++j)
+        {
+            double lj = 1.0;
+            for (int i = 0; i <= order; ++i)
+            {
+                if (i == j) continue;
+                const int x = fi[idir], xj = i0 + j, xi = i0 + i;
+                lj = lj*(x - xi)/(xj - xi);
+            }
+            int src[3] = {fi[0], fi[1], fi[2]};
+            src[idir] = i0 + j;
+            const double* e = a + G.cell(src[0], src[1], src[2], lb);
+            for (int v = 0; v < 5; ++v) gv[v] = gv[v] + lj*e[v];
+        }
+        for (int v = 0; v < 5; ++v) dst[v] = gv[v];
+    });
+    return 0;
+}
+
+// A boundary_fill kernel (boundary_fill.h:99-127) of the affine family
+// ghost[v] = dom[v] q_image[v] + fill[v] q_ghost[v] + add[v], the velocity
+// component normal to the face taking the *_normal coefficients; a term with
+// a zero coefficient is left out, the others are rounded in that order.
+int ora_boundary_fill_affine(const ora_grid* g, double* a, int face_mask, const ora_fill_kernel* kern)
+{
+    if (!g || !a || !kern) return -1;
+    const Geo G(g);
+    fill_faces(G, a, face_mask, [&](int lb, int idir, int pm, const int fi[3], double* dst)
+    {
+        int im[3] = {fi[0], fi[1], fi[2]};
+        if (pm == 0) im[idir] = -1 - fi[idir];
+        else         im[idir] = 2*G.g.nc[idir] - (fi[idir] + 1);
+        const double* d = a + G.cell(im[0], im[1], im[2], lb);
+        double out[5];
+        for (int v = 0; v < 5; ++v)
+        {
+            const bool nrm = (v == 2 + idir);
+            const double cd = nrm ? kern->dom_normal : kern->dom[v];
+            const double cf = nrm ? kern->fill_normal : kern->fill[v];
+            const double ca = nrm ? kern->add_normal : kern->add[v];
+            double r = 0.0;
+            bool any = false;
+            if (cd != 0.0) { r = cd*d[v]; any = true; }
+            if (cf != 0.0) { const double t = cf*dst[v]; r = any ? r + t : t; any = true; }
+            if (ca != 0.0) { r = any ? r + ca : ca; any = true; }
+            out[v] = r;
+        }
+        for (int v = 0; v < 5; ++v) dst[v] = out[v];
+    });
     return 0;
 }
 

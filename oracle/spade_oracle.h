@@ -64,6 +64,17 @@ int ora_flux_div_blocks(const ora_grid* g, const ora_gas* gas, const ora_flux* f
                         int increment, const int* blocks, int nblocks);
 int ora_exchange(const ora_grid* g, double* a);
 int ora_boundary_fill_wall(const ora_grid* g, double* a, int face, double t_wall);
+/* boundary_fill(arr, faces, extrapolate<order>); face_mask bit 2*dir+pm */
+int ora_boundary_fill_extrap(const ora_grid* g, double* a, int face_mask, int order);
+/* boundary_fill(arr, faces, kernel) for an affine kernel: per primitive v,
+ * ghost = dom[v] image + fill[v] ghost + add[v]; the face-normal velocity
+ * component takes the *_normal coefficients */
+typedef struct
+{
+    double dom[5], fill[5], add[5];
+    double dom_normal, fill_normal, add_normal;
+} ora_fill_kernel;
+int ora_boundary_fill_affine(const ora_grid* g, double* a, int face_mask, const ora_fill_kernel* kern);
 int ora_prim_to_cons(const ora_gas* gas, const double* prim, double* cons, long ncell);
 int ora_cons_to_prim(const ora_gas* gas, const double* cons, double* prim, long ncell);
 int ora_ssprk3_step(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, const ora_bc* bc,

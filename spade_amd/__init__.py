@@ -37,7 +37,8 @@ from ._abi import SpadeError, check, load
 __all__ = [
     "Context", "CartesianGrid", "GridArray", "IdealGas", "cent_keep", "totani_lr", "pressure_diss_lr",
     "weno", "fweno", "weno_fds", "first_order", "visc_lr", "constant_viscosity", "power_law", "const_sensor",
-    "ducros", "compose", "flux_div", "create_exchange", "boundary_fill_wall", "Integrator",
+    "ducros", "compose", "flux_div", "create_exchange", "boundary_fill_wall", "boundary_fill", "extrapolate",
+    "FillKernel", "isothermal_wall", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "Integrator",
     "ssprk3", "ssprk3_opt", "max_wavespeed", "tanh_map", "SpadeError", "diagnostics", "CaseSpec", "Batch",
     "grid_desc", "output_vtk", "output_vtk_host", "binary_write", "binary_read", "PRIM_NAMES", "CONS_NAMES",
     "FLUX_NAMES",
@@ -376,6 +377,51 @@ def create_exchange(arr, periodic=None):
 def boundary_fill_wall(arr, face, t_wall):
     """Isothermal no-slip mirror ghosts on face = 2*dir + (0 min | 1 max)."""
     check(load().spd_boundary_fill_wall(arr._h, int(face), float(t_wall)))
+
+
+# boundary identifiers (grid/boundary_fill.h:9-15), combined with |
+xmin, xmax, ymin, ymax, zmin, zmax = 1, 2, 4, 8, 16, 32
+
+
+class extrapolate:
+    """boundary::extrapolate<order> (grid/boundary_fill.h:17-26)."""
+
+    def __init__(self, order):
+        self.order = int(order)
+
+
+class FillKernel:
+    """A boundary_fill kernel kern(q_image, q_ghost, x, idir) given as data:
+    per primitive v (p, T, u, v, w), ghost = dom[v] q_image + fill[v] q_ghost
+    + add[v]; the velocity component normal to the face takes dom_normal /
+    fill_normal / add_normal (default: the u entries).  Zero coefficients
+    leave their term out (grid/boundary_fill.h:99-127)."""
+
+    def __init__(self, dom, fill=(0.0,) * 5, add=(0.0,) * 5, dom_normal=None, fill_normal=None, add_normal=None):
+        k = _abi.FillKernel()
+        for v in range(5):
+            k.dom[v], k.fill[v], k.add[v] = float(dom[v]), float(fill[v]), float(add[v])
+        k.dom_normal = float(dom[2] if dom_normal is None else dom_normal)
+        k.fill_normal = float(fill[2] if fill_normal is None else fill_normal)
+        k.add_normal = float(add[2] if add_normal is None else add_normal)
+        self._k = k
+
+
+def isothermal_wall(t_wall):
+    """No-slip isothermal mirror: (p, 2 T_wall - T, -u, -v, -w) of the image cell."""
+    return FillKernel(dom=(1.0, -1.0, -1.0, -1.0, -1.0), add=(0.0, 2.0 * t_wall, 0.0, 0.0, 0.0))
+
+
+def boundary_fill(arr, faces, kernel):
+    """algs::boundary_fill(arr, boundaries, kern) (grid/boundary_fill.h:32-133):
+    faces = xmin | ymax | ..., filled in face order; kernel = extrapolate(order)
+    or a FillKernel."""
+    if isinstance(kernel, extrapolate):
+        check(load().spd_boundary_fill_extrap(arr._h, int(faces), kernel.order))
+    elif isinstance(kernel, FillKernel):
+        check(load().spd_boundary_fill(arr._h, int(faces), C.byref(kernel._k)))
+    else:
+        raise TypeError("boundary_fill kernel must be extrapolate(order) or FillKernel(...)")
 
 
 ssprk3 = _abi.SCHEME_SSPRK3

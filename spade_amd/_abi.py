@@ -39,6 +39,11 @@ class FluxTerm(C.Structure):
                 ("prandtl", C.c_double), ("diss_kind", C.c_int), ("full", C.c_int)]
 
 
+class FillKernel(C.Structure):
+    _fields_ = [("dom", C.c_double * 5), ("fill", C.c_double * 5), ("add", C.c_double * 5),
+                ("dom_normal", C.c_double), ("fill_normal", C.c_double), ("add_normal", C.c_double)]
+
+
 class FluxDesc(C.Structure):
     _fields_ = [("nterms", C.c_int), ("terms", FluxTerm * MAX_TERMS), ("gamma", C.c_double), ("R", C.c_double)]
 
@@ -82,6 +87,8 @@ SIGNATURES = {
     "spd_flux_div_kernel": (C.c_int, [_vp, C.POINTER(FluxDesc), C.POINTER(C.c_int)]),
     "spd_exchange": (C.c_int, [_vp]),
     "spd_boundary_fill_wall": (C.c_int, [_vp, C.c_int, C.c_double]),
+    "spd_boundary_fill_extrap": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "spd_boundary_fill": (C.c_int, [_vp, C.c_int, C.POINTER(FillKernel)]),
     "spd_integrator_create": (C.c_int, [_vp, C.POINTER(FluxDesc), C.POINTER(BCDesc), C.c_int, C.c_double,
                                         C.POINTER(_vp)]),
     "spd_integrator_advance": (C.c_int, [_vp, C.c_int]),

@@ -419,6 +419,35 @@ int spd_boundary_fill_wall(spd_array_t arr, int face, double t_wall)
     });
 }
 
+int spd_boundary_fill_extrap(spd_array_t arr, int face_mask, int order)
+{
+    return guarded([&]
+    {
+        require(arr != nullptr, "null argument");
+        require(face_mask >= 0 && face_mask < 64, "face mask must be a subset of the six faces (0..63)");
+        require(order >= 0, "extrapolation order must be >= 0");
+        const DevGeom& G = arr->a.grid->geom;
+        for (int d = 0; d < 3; ++d)
+            if (face_mask & (3 << (2*d)))
+                require(order + 1 <= G.nc[d], "extrapolation needs order + 1 <= cells per block along the face normal");
+        launch_fill_extrap(*arr->a.grid, arr->a.d, face_mask, order, arr->a.grid->ctx->stream);
+    });
+}
+
+int spd_boundary_fill(spd_array_t arr, int face_mask, const spd_fill_kernel* kern)
+{
+    return guarded([&]
+    {
+        require(arr != nullptr && kern != nullptr, "null argument");
+        require(face_mask >= 0 && face_mask < 64, "face mask must be a subset of the six faces (0..63)");
+        const DevGeom& G = arr->a.grid->geom;
+        for (int d = 0; d < 3; ++d)
+            if (face_mask & (3 << (2*d)))
+                require(G.ng <= G.nc[d], "mirror fill needs ng <= cells per block along the face normal");
+        launch_fill_affine(*arr->a.grid, arr->a.d, face_mask, *kern, arr->a.grid->ctx->stream);
+    });
+}
+
 int spd_integrator_create(spd_array_t q, const spd_flux_desc* flux, const spd_bc_desc* bc,
                           int scheme, double dt, spd_integrator_t* out)
 {

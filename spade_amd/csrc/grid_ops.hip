@@ -294,6 +294,87 @@ void diagnostics(Grid& g, const double* q, double R, double& ke, double& en, hip
     ke = a/n; en = b/n;
 }
 
+// Ghost cell c of face (idir, pm) of face block blockIdx.y: its block and
+// index (normal index outside [0, n), tangential indices over [-ng, n+ng)).
+// false for threads past the face's ghost cells.
+__device__ inline bool face_ghost(const DevGeom& G, int idir, int pm, int& lb, int idx[3])
+{
+    const int d1 = idir == 0 ? 1 : 0, d2 = idir == 2 ? 1 : 2;
+    int b[3];
+    b[idir] = pm == 0 ? 0 : G.nb[idir] - 1;
+    b[d1] = blockIdx.y % G.nb[d1];
+    b[d2] = blockIdx.y / G.nb[d1];
+    lb = b[0] + G.nb[0]*(b[1] + G.nb[1]*b[2]);
+    int ext[3] = {G.N[0], G.N[1], G.N[2]};
+    ext[idir] = G.ng;
+    const int c = blockIdx.x*blockDim.x + threadIdx.x;
+    if (c >= ext[0]*ext[1]*ext[2]) return false;
+    idx[0] = c % ext[0] - G.ng; idx[1] = (c / ext[0]) % ext[1] - G.ng; idx[2] = c / (ext[0]*ext[1]) - G.ng;
+    if (pm == 1) idx[idir] = G.nc[idir] + (idx[idir] + G.ng);           // [n, n+ng); pm = 0: [-ng, 0)
+    return true;
+}
+
+// boundary_fill with extrap_t<order> (grid/boundary_fill.h:17-26, 63-98):
+// Lagrange extrapolation along the face normal through cells i0 .. i0+order
+// (i0 = 0 at a min face, n - order - 1 at a max face), the weights built on
+// integer cell indices as l_j = l_j (x - x_i) / (x_j - x_i) over ascending
+// i != j, the ghost accumulated as g = g + l_j q_j from 0.
+__global__ void __launch_bounds__(256)
+k_fill_extrap(DevGeom G, double* __restrict__ a, int idir, int pm, int order)
+{
+    int lb, idx[3];
+    if (!face_ghost(G, idir, pm, lb, idx)) return;
+    const int i0 = pm*(G.nc[idir] - order - 1);
+    double gv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j <= order; ++j)
+    {
+        double lj = 1.0;
+        for (int i = 0; i <= order; ++i)
+        {
+            if (i == j) continue;
+            const int x = idx[idir], xj = i0 + j, xi = i0 + i;
+            lj = lj*(x - xi)/(xj - xi);
+        }
+        int src[3] = {idx[0], idx[1], idx[2]};
+        src[idir] = i0 + j;
+        #pragma unroll
+        for (int v = 0; v < 5; ++v) gv[v] = gv[v] + lj*a[cell_index(G, lb, v, src[0], src[1], src[2])];
+    }
+    #pragma unroll
+    for (int v = 0; v < 5; ++v) a[cell_index(G, lb, v, idx[0], idx[1], idx[2])] = gv[v];
+}
+
+// boundary_fill with a kernel of the affine family (boundary_fill.h:99-127):
+// ghost[v] = dom[v] q_image[v] + fill[v] q_ghost[v] + add[v], the velocity
+// component normal to the face taking the *_normal coefficients; terms with a
+// zero coefficient are left out, the others rounded in that order.  The image
+// cell mirrors the ghost across the face (-1 - i, or 2n - (i + 1)).
+__global__ void __launch_bounds__(256)
+k_fill_affine(DevGeom G, double* __restrict__ a, int idir, int pm, spd_fill_kernel K)
+{
+    int lb, idx[3];
+    if (!face_ghost(G, idir, pm, lb, idx)) return;
+    int im[3] = {idx[0], idx[1], idx[2]};
+    im[idir] = (pm == 0) ? (-1 - idx[idir]) : (2*G.nc[idir] - (idx[idir] + 1));
+    double out[5];
+    #pragma unroll
+    for (int v = 0; v < 5; ++v)
+    {
+        const bool nrm = (v == 2 + idir);
+        const double cd = nrm ? K.dom_normal : K.dom[v];
+        const double cf = nrm ? K.fill_normal : K.fill[v];
+        const double ca = nrm ? K.add_normal : K.add[v];
+        double r = 0.0;
+        bool any = false;
+        if (cd != 0.0) { r = cd*a[cell_index(G, lb, v, im[0], im[1], im[2])]; any = true; }
+        if (cf != 0.0) { const double t = cf*a[cell_index(G, lb, v, idx[0], idx[1], idx[2])]; r = any ? r + t : t; any = true; }
+        if (ca != 0.0) { r = any ? r + ca : ca; }
+        out[v] = r;
+    }
+    #pragma unroll
+    for (int v = 0; v < 5; ++v) a[cell_index(G, lb, v, idx[0], idx[1], idx[2])] = out[v];
+}
+
 void launch_exchange(const Grid& g, double* a, hipStream_t s)
 {
     const int items = exchange_items(g.geom);
@@ -312,6 +393,40 @@ void launch_wall_fill(const Grid& g, double* a, int face, double t_wall, hipStre
     const int face_blocks = G.nb[d1]*G.nb[d2];
     hipLaunchKernelGGL(k_wall_fill, dim3((n + 255)/256, face_blocks), dim3(256), 0, s, G, a, idir, pm, t_wall);
     SPD_LAUNCH_CHECK();
+}
+
+// grid of one boundary face: ghost cells per face block x face blocks
+static void face_grid(const DevGeom& G, int idir, dim3& grid)
+{
+    int ext[3] = {G.N[0], G.N[1], G.N[2]};
+    ext[idir] = G.ng;
+    const int n = ext[0]*ext[1]*ext[2];
+    const int d1 = idir == 0 ? 1 : 0, d2 = idir == 2 ? 1 : 2;
+    grid = dim3((n + 255)/256, G.nb[d1]*G.nb[d2]);
+}
+
+void launch_fill_extrap(const Grid& g, double* a, int face_mask, int order, hipStream_t s)
+{
+    for (int face = 0; face < 6; ++face)
+    {
+        if (!(face_mask & (1 << face))) continue;
+        dim3 grid;
+        face_grid(g.geom, face/2, grid);
+        hipLaunchKernelGGL(k_fill_extrap, grid, dim3(256), 0, s, g.geom, a, face/2, face % 2, order);
+        SPD_LAUNCH_CHECK();
+    }
+}
+
+void launch_fill_affine(const Grid& g, double* a, int face_mask, const spd_fill_kernel& k, hipStream_t s)
+{
+    for (int face = 0; face < 6; ++face)
+    {
+        if (!(face_mask & (1 << face))) continue;
+        dim3 grid;
+        face_grid(g.geom, face/2, grid);
+        hipLaunchKernelGGL(k_fill_affine, grid, dim3(256), 0, s, g.geom, a, face/2, face % 2, k);
+        SPD_LAUNCH_CHECK();
+    }
 }
 
 void launch_aos_to_soa(const Grid& g, const double* stage, double* a, int lb0, int nb, hipStream_t s)

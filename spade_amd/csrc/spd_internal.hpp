@@ -114,6 +114,9 @@ int flux_div_spec_code(const spd_flux_desc& f);
 bool flux_has_hybrid(const spd_flux_desc& f);
 void launch_exchange(const Grid& g, double* a, hipStream_t s);
 void launch_wall_fill(const Grid& g, double* a, int face, double t_wall, hipStream_t s);
+// boundary_fill over the faces of face_mask (bit 2*dir+pm), one launch per face in face order
+void launch_fill_extrap(const Grid& g, double* a, int face_mask, int order, hipStream_t s);
+void launch_fill_affine(const Grid& g, double* a, int face_mask, const spd_fill_kernel& k, hipStream_t s);
 void launch_fill(const Grid& g, double* a, double value, hipStream_t s);
 void launch_aos_to_soa(const Grid& g, const double* stage, double* a, int lb0, int nb, hipStream_t s);
 void launch_soa_to_aos(const Grid& g, const double* a, double* stage, int lb0, int nb, hipStream_t s);

@@ -107,3 +107,13 @@ def test_hybrid_refused_on_the_device():
             sa.flux_div(q, rhs, RawFlux(terms, GAMMA, R), increment=False)
         assert e.value.code == 3 and "hybrid" in str(e.value), str(e.value)
     assert np.all(rhs.download() == 7.0)
+
+
+def test_boundary_fill_arguments_refused():
+    sa = _sa()
+    g, q, rhs = _arrays(sa)               # 8^3 cells per block
+    before = q.download()
+    _refused(sa, lambda: sa.boundary_fill(q, 64, sa.extrapolate(1)), "face mask")
+    _refused(sa, lambda: sa.boundary_fill(q, sa.xmin, sa.extrapolate(8)), "order + 1")
+    _refused(sa, lambda: sa.boundary_fill(q, sa.xmin, sa.extrapolate(-1)), "order")
+    assert np.array_equal(q.download(), before)

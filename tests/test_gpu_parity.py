@@ -46,6 +46,49 @@ def test_wall_exchange_and_fill_bitwise():
     assert np.array_equal(a.download(), gi.load("g3_prim_wall_ghost"))
 
 
+def _g4(sa):
+    g = grid(G1, periodic=(False, False, False))
+    a = sa.GridArray(g).upload(gi.load("g3_prim_init_ghost"))
+    sa.create_exchange(a).exchange(a)
+    assert np.array_equal(a.download(), gi.load("g4_prim_exch_ghost"))
+    return a
+
+
+@pytest.mark.parametrize("order, faces, name", [
+    (2, "all", "g4_prim_extrap2_ghost"), (0, "all", "g4_prim_extrap0_ghost"),
+    (3, "xmin|ymax", "g4_prim_extrap3_xmin_ymax_ghost")])
+def test_boundary_fill_extrapolation_bitwise(order, faces, name):
+    """boundary_fill(arr, faces, extrapolate<order>) on a grid with no periodic
+    axis, against the compiled reference (faces in order: edges and corners
+    read the ghosts of the faces filled before them)."""
+    sa = _sa()
+    a = _g4(sa)
+    mask = (sa.xmin | sa.xmax | sa.ymin | sa.ymax | sa.zmin | sa.zmax) if faces == "all" else (sa.xmin | sa.ymax)
+    sa.boundary_fill(a, mask, sa.extrapolate(order))
+    assert np.array_equal(a.download(), gi.load(name))
+
+
+def test_boundary_fill_user_kernel_bitwise():
+    """The reference driver's four-argument slip-wall lambda (p blended with
+    the ghost's own value, isothermal T, normal velocity mirrored) as a data
+    kernel on all six faces, against the compiled reference."""
+    sa = _sa()
+    a = _g4(sa)
+    t = P["t_wall"]
+    k = sa.FillKernel(dom=(1.0, -1.0, 1.0, 1.0, 1.0), fill=(0.25, 0.0, 0.0, 0.0, 0.0), add=(0.0, 2.0 * t, 0.0, 0.0, 0.0),
+                      dom_normal=-1.0, fill_normal=0.0, add_normal=0.0)
+    sa.boundary_fill(a, sa.xmin | sa.xmax | sa.ymin | sa.ymax | sa.zmin | sa.zmax, k)
+    assert np.array_equal(a.download(), gi.load("g4_prim_slip_ghost"))
+
+
+def test_boundary_fill_isothermal_kernel_equals_wall_fill():
+    sa = _sa()
+    g = grid(G1, periodic=(True, False, True))
+    a = sa.GridArray(g).upload(gi.load("g3_prim_exch_ghost"))
+    sa.boundary_fill(a, sa.ymin | sa.ymax, sa.isothermal_wall(P["t_wall"]))
+    assert np.array_equal(a.download(), gi.load("g3_prim_wall_ghost"))
+
+
 @pytest.mark.parametrize("name", sorted(FLUX_CASES))
 def test_flux_div(name):
     if name in GPU_UNSUPPORTED:

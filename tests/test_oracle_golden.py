@@ -64,6 +64,51 @@ def test_wall_exchange_and_boundary_fill_bitwise():
     assert np.array_equal(gi.interior(rhs, 2), gi.load("g3_rhs_keep2_visc_const"))
 
 
+ALL_FACES = 0b111111
+
+
+def _g4():
+    """The non-periodic grid after its exchange (the state the fills start from)."""
+    g = _grid(G1, periodic=(0, 0, 0))
+    a = gi.load("g3_prim_init_ghost").copy()       # same initial values as the G3 grid
+    po.exchange(g, a)
+    assert np.array_equal(a, gi.load("g4_prim_exch_ghost"))
+    return g, a
+
+
+@pytest.mark.parametrize("order, mask, name", [(2, ALL_FACES, "g4_prim_extrap2_ghost"),
+                                                (0, ALL_FACES, "g4_prim_extrap0_ghost"),
+                                                (3, 0b001001, "g4_prim_extrap3_xmin_ymax_ghost")])
+def test_boundary_fill_extrapolation_bitwise(order, mask, name):
+    g, a = _g4()
+    po.boundary_fill_extrap(g, a, mask, order)
+    assert np.array_equal(a, gi.load(name))
+
+
+def slip_kernel(t_wall):
+    """The reference driver's four-argument slip-wall kernel as an affine
+    kernel: p = 0.25 p_ghost + p_image, T = 2 T_wall - T_image, normal
+    velocity mirrored, tangential velocity copied."""
+    return po.fill_kernel(dom=(1.0, -1.0, 1.0, 1.0, 1.0), fill=(0.25, 0.0, 0.0, 0.0, 0.0),
+                          add=(0.0, 2.0 * t_wall, 0.0, 0.0, 0.0), dom_normal=-1.0, fill_normal=0.0, add_normal=0.0)
+
+
+def test_boundary_fill_user_kernel_bitwise():
+    g, a = _g4()
+    po.boundary_fill_affine(g, a, ALL_FACES, slip_kernel(P["t_wall"]))
+    assert np.array_equal(a, gi.load("g4_prim_slip_ghost"))
+
+
+def test_boundary_fill_wall_kernel_is_affine():
+    """The isothermal no-slip mirror written as an affine kernel gives the
+    fixed wall fill's bits."""
+    g = _grid(G1, periodic=(1, 0, 1))
+    a = gi.load("g3_prim_exch_ghost").copy()
+    k = po.fill_kernel(dom=(1.0, -1.0, -1.0, -1.0, -1.0), add=(0.0, 2.0 * P["t_wall"], 0.0, 0.0, 0.0))
+    po.boundary_fill_affine(g, a, 0b001100, k)
+    assert np.array_equal(a, gi.load("g3_prim_wall_ghost"))
+
+
 def _g2_setup():
     g = _grid(G2)
     q = gi.embed(gi.load("g2_prim0"), 2)
