@@ -203,6 +203,73 @@ int spd_integrator_destroy(spd_integrator_t it);
 /* Residual array k_i of the integrator (0-based), for inspection. */
 spd_array_t spd_integrator_residual(spd_integrator_t it, int i);
 
+/* ---- explicit Runge-Kutta with any Butcher table and caller-supplied
+ *      residual / boundary callbacks: integrator_t(axis, scheme, data, rhs, bc,
+ *      trans).advance() (time-integration/integrator.h:18-59, advance.h:107-280,
+ *      explicit.h:27-116) ---- */
+#define SPD_RK_MAX_STAGES 6
+typedef struct { long num, den; } spd_ratio;       /* std::ratio<num, den>; den = 0 reads as 0 */
+typedef struct {
+    int       stages;
+    int       high_storage;                         /* rk_t<..., true>: a second solution copy (var_size 2) */
+    spd_ratio a[SPD_RK_MAX_STAGES][SPD_RK_MAX_STAGES];/* table rows: a[i][j], j < i */
+    spd_ratio b[SPD_RK_MAX_STAGES];                 /* accumulation (accum_type) */
+    spd_ratio c[SPD_RK_MAX_STAGES];                 /* stage times (dt_type) */
+} spd_rk_table;
+/* The tables explicit.h defines: rk2_t, rk2hs_t, rk4_t, ssprk34_t, ssprk3_t, ssprk3hs_t, rk38r_t. */
+enum { SPD_RK2 = 0, SPD_RK2HS, SPD_RK4, SPD_SSPRK34, SPD_SSPRK3_TABLE, SPD_SSPRK3HS, SPD_RK38R };
+int spd_rk_table_builtin(int which, spd_rk_table* out);
+
+/* The state transform of the integrator: state_transform_t(cons_t, gas) (primitive
+ * storage, conserved stage arithmetic; navier-stokes/state_transformation.h:37-66),
+ * which selects the transform-aware advance (advance.h:233-280), or
+ * identity_transform (integrator.h:8-14), which selects the generic advance
+ * (advance.h:107-230) on whatever variables the array stores. */
+enum { SPD_TRANS_PRIM_CONS = 0, SPD_TRANS_IDENTITY = 1 };
+/* rhs(r&, const q&, t) and bc(q&, t) of integrator_t; a non-zero return stops the
+ * step with SPD_ERR_ARG.  They are called on the integrator's own host thread and
+ * may call any function of this library on the arrays they are given. */
+typedef int (*spd_rhs_fn)(spd_array_t resid, spd_array_t sol, double t, void* user);
+typedef int (*spd_bc_fn)(spd_array_t sol, double t, void* user);
+typedef struct {
+    spd_rk_table        table;
+    int                 trans;          /* SPD_TRANS_* */
+    double              gamma, R;       /* gas of SPD_TRANS_PRIM_CONS */
+    double              t0, dt;         /* time_axis_t(t0, dt) */
+    spd_rhs_fn          rhs;            /* NULL: flux_div(sol, resid, *flux, overwrite) */
+    spd_bc_fn           bc;             /* NULL: exchange + the isothermal walls of *walls (walls may be NULL) */
+    void*               user;
+    const spd_flux_desc* flux;          /* for the default rhs */
+    const spd_bc_desc*  walls;          /* for the default bc */
+} spd_rk_desc;
+/* An integrator on q (advanced in place) with stages residual arrays (and a second
+ * solution copy for a high-storage table with the identity transform);
+ * spd_integrator_advance / _destroy / _residual apply to it. */
+int spd_rk_integrator_create(spd_array_t q, const spd_rk_desc* desc, spd_integrator_t* out);
+/* Current time of the integrator's axis (t0 + steps * dt, summed as the reference does). */
+int spd_integrator_time(spd_integrator_t it, double* t);
+
+/* One transform-aware stage update (transform_advance_to, advance.h:56-102) on the
+ * interior cells of prims: cons = p2c(q); cons += coef[j] * resid[j] for each
+ * non-zero coef[j] in order; q = c2p(cons), where coef[j] is the caller's
+ * (a_ij - a_{i-1,j}) dt.  push_ghosts = 1 also writes each updated cell into the
+ * ghost cells of the blocks it feeds (what spd_exchange would copy afterwards; the
+ * domain faces of non-periodic axes are left for the boundary fill). */
+int spd_rk_stage(spd_array_t prims, const spd_array_t* resid, const double* coef, int n, double gamma, double R,
+                 int push_ghosts);
+
+/* Element-wise array operations of grid_array (grid/grid_array.h:286-396): scale
+ * every stored cell (operator*=(scalar)); dst += src / dst -= src on the interior
+ * cells (operator+= / -=, support_of(exclude_exchanges)). */
+int spd_array_scale(spd_array_t arr, double s);
+int spd_array_add(spd_array_t dst, spd_array_t src);
+int spd_array_sub(spd_array_t dst, spd_array_t src);
+/* convert_state in place (navier-stokes/fluid_state.h:102-135): SPD_TO_CONS
+ * (prim -> cons) or SPD_TO_PRIM (cons -> prim), on every stored cell
+ * (include_ghosts = 1, the state transform's include_exchanges) or the interior. */
+enum { SPD_TO_CONS = 0, SPD_TO_PRIM = 1 };
+int spd_array_convert(spd_array_t arr, int direction, double gamma, double R, int include_ghosts);
+
 /* ---- reductions on interior cells (transform_reduce, algs/transform_reduce.h:53-190) ---- */
 /* max(|u| + sqrt(gamma R T)): the CFL wave speed (development/cuda-tgv/main.cc:155-172) */
 int spd_max_wavespeed(spd_array_t prims, double gamma, double R, double* out);

@@ -4,6 +4,7 @@
 // except::sp_exception, core/except.h:12-30).
 #pragma once
 #include <cstring>
+#include <functional>
 #include <initializer_list>
 #include <stdexcept>
 #include <string>
@@ -70,12 +71,19 @@ class grid_array
 {
     spd_array_t h_ = nullptr;
     const grid* g_;
+    bool owner_ = true;
 public:
     explicit grid_array(const grid& g) : g_(&g) { check(spd_array_create(g.handle(), &h_)); }
-    ~grid_array() { if (h_) spd_array_destroy(h_); }
+    // a non-owning view of an array the library hands to a callback
+    grid_array(const grid& g, spd_array_t view) : h_(view), g_(&g), owner_(false) {}
+    ~grid_array() { if (h_ && owner_) spd_array_destroy(h_); }
     grid_array(const grid_array&) = delete;
     grid_array& operator=(const grid_array&) = delete;
     grid_array& operator=(double v) { check(spd_array_fill(h_, v)); return *this; }
+    // element-wise operations (grid/grid_array.h:286-396)
+    grid_array& operator*=(double s) { check(spd_array_scale(h_, s)); return *this; }
+    grid_array& operator+=(const grid_array& o) { check(spd_array_add(h_, o.h_)); return *this; }
+    grid_array& operator-=(const grid_array& o) { check(spd_array_sub(h_, o.h_)); return *this; }
     void upload(const double* host) { check(spd_array_upload(h_, host)); }
     void download(double* host) const { check(spd_array_download(h_, host)); }
     std::vector<double> download() const { std::vector<double> v(g_->host_size()); download(v.data()); return v; }
@@ -244,6 +252,55 @@ public:
     integrator_t(const integrator_t&) = delete;
     integrator_t& operator=(const integrator_t&) = delete;
     void advance(int nsteps = 1) { check(spd_integrator_advance(h_, nsteps)); }
+};
+
+// integrator_t with any explicit table and the caller's residual and boundary
+// (time-integration/integrator.h:18-59, advance.h:107-280): trans =
+// state_transform (primitive storage) or identity_transform.  An empty rhs is
+// flux_div(sol, resid, f); an empty bc the exchange plus the walls.
+enum transform { state_transform = SPD_TRANS_PRIM_CONS, identity_transform = SPD_TRANS_IDENTITY };
+inline spd_rk_table rk_table(int which) { spd_rk_table t; check(spd_rk_table_builtin(which, &t)); return t; }
+class rk_integrator_t
+{
+public:
+    using rhs_fn = std::function<void(grid_array& resid, const grid_array& sol, double t)>;
+    using bc_fn = std::function<void(grid_array& sol, double t)>;
+private:
+    spd_integrator_t h_ = nullptr;
+    const grid* g_;
+    rhs_fn rhs_;
+    bc_fn bc_;
+    static int call_rhs(spd_array_t r, spd_array_t q, double t, void* u)
+    {
+        auto* self = static_cast<rk_integrator_t*>(u);
+        try { grid_array rr(*self->g_, r), qq(*self->g_, q); self->rhs_(rr, qq, t); return 0; }
+        catch (...) { return 1; }
+    }
+    static int call_bc(spd_array_t q, double t, void* u)
+    {
+        auto* self = static_cast<rk_integrator_t*>(u);
+        try { grid_array qq(*self->g_, q); self->bc_(qq, t); return 0; }
+        catch (...) { return 1; }
+    }
+public:
+    rk_integrator_t(grid_array& q, const spd_rk_table& table, double dt, transform trans, const ideal_gas_t& gas,
+                    rhs_fn rhs = {}, bc_fn bc = {}, const flux* f = nullptr, double t0 = 0.0)
+        : g_(&q.get_grid()), rhs_(std::move(rhs)), bc_(std::move(bc))
+    {
+        spd_rk_desc d;
+        std::memset(&d, 0, sizeof(d));
+        d.table = table; d.trans = trans; d.gamma = gas.gamma; d.R = gas.R; d.t0 = t0; d.dt = dt;
+        d.rhs = rhs_ ? &call_rhs : nullptr;
+        d.bc = bc_ ? &call_bc : nullptr;
+        d.user = this;
+        d.flux = f ? &f->desc() : nullptr;
+        check(spd_rk_integrator_create(q.handle(), &d, &h_));
+    }
+    ~rk_integrator_t() { if (h_) spd_integrator_destroy(h_); }
+    rk_integrator_t(const rk_integrator_t&) = delete;
+    rk_integrator_t& operator=(const rk_integrator_t&) = delete;
+    void advance(int nsteps = 1) { check(spd_integrator_advance(h_, nsteps)); }
+    double time() const { double t = 0.0; check(spd_integrator_time(h_, &t)); return t; }
 };
 
 inline double max_wavespeed(const grid_array& q, const ideal_gas_t& gas)

@@ -52,6 +52,47 @@ class FillKernel(C.Structure):
                 ("dom_normal", C.c_double), ("fill_normal", C.c_double), ("add_normal", C.c_double)]
 
 
+ORA_RK_MAX = 6
+
+
+class RKTable(C.Structure):
+    _fields_ = [("stages", C.c_int), ("high_storage", C.c_int),
+                ("a", (C.c_long * 2) * ORA_RK_MAX * ORA_RK_MAX), ("b", (C.c_long * 2) * ORA_RK_MAX),
+                ("c", (C.c_long * 2) * ORA_RK_MAX)]
+
+
+# explicit Butcher tables of time-integration/explicit.h:41-116 as (num, den):
+# rows a (j < i), accumulation b, stage times c, high storage
+_H = (1, 2)
+RK_TABLES = {
+    "rk2": ([[], [_H]], [(0, 1), (1, 1)], [(0, 1), _H], False),
+    "rk2hs": ([[], [_H]], [(0, 1), (1, 1)], [(0, 1), _H], True),
+    "rk4": ([[], [_H], [(0, 1), _H], [(0, 1), (0, 1), (1, 1)]], [(1, 6), (1, 3), (1, 3), (1, 6)],
+            [(0, 1), _H, _H, (1, 1)], False),
+    "ssprk34": ([[], [_H], [_H, _H], [(1, 6), (1, 6), (1, 6)]], [(1, 6), (1, 6), (1, 6), _H],
+                [(0, 1), _H, (1, 1), _H], False),
+    "ssprk3": ([[], [(1, 1)], [(1, 4), (1, 4)]], [(1, 6), (1, 6), (2, 3)], [(0, 1), (1, 1), _H], False),
+    "ssprk3hs": ([[], [(1, 1)], [(1, 4), (1, 4)]], [(1, 6), (1, 6), (2, 3)], [(0, 1), (1, 1), _H], True),
+    "rk38r": ([[], [(1, 3)], [(-1, 3), (1, 1)], [(1, 1), (-1, 1), (1, 1)]], [(1, 8), (3, 8), (3, 8), (1, 8)],
+              [(0, 1), (1, 3), (2, 3), (1, 1)], False),
+}
+
+
+def rk_table(name):
+    a, b, c, hs = RK_TABLES[name]
+    t = RKTable()
+    t.stages = len(b)
+    t.high_storage = int(hs)
+    for i, row in enumerate(a):
+        for j, (n, d) in enumerate(row):
+            t.a[i][j][0], t.a[i][j][1] = n, d
+    for i, (n, d) in enumerate(b):
+        t.b[i][0], t.b[i][1] = n, d
+    for i, (n, d) in enumerate(c):
+        t.c[i][0], t.c[i][1] = n, d
+    return t
+
+
 class BC(C.Structure):
     _fields_ = [("wall_mask", C.c_int), ("t_wall", C.c_double)]
 
@@ -69,6 +110,10 @@ def _load():
                                         C.POINTER(C.c_int), C.c_int]
     lib.ora_exchange.argtypes = [C.POINTER(Grid), dp]
     lib.ora_boundary_fill_wall.argtypes = [C.POINTER(Grid), dp, C.c_int, C.c_double]
+    lib.ora_rk_transform_step.argtypes = [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Flux), C.POINTER(BC),
+                                          C.POINTER(RKTable), C.c_double, dp, C.POINTER(dp)]
+    lib.ora_rk_identity_step.argtypes = [C.POINTER(Grid), C.POINTER(Gas), C.POINTER(Flux), C.POINTER(RKTable),
+                                         C.c_double, dp, dp, C.POINTER(dp), dp]
     lib.ora_boundary_fill_extrap.argtypes = [C.POINTER(Grid), dp, C.c_int, C.c_int]
     lib.ora_boundary_fill_affine.argtypes = [C.POINTER(Grid), dp, C.c_int, C.POINTER(FillKernel)]
     lib.ora_prim_to_cons.argtypes = [C.POINTER(Gas), dp, dp, C.c_long]
@@ -165,6 +210,32 @@ def exchange(g, a):
 def boundary_fill_wall(g, a, face, t_wall):
     lib().ora_boundary_fill_wall(C.byref(g), _ptr(a), face, t_wall)
     return a
+
+
+_dp = C.POINTER(C.c_double)
+
+
+def rk_transform_step(g, gas, flux, bc, table, dt, q, steps=1):
+    """Explicit table with the prim<->cons transform (advance.h:233-280)."""
+    res = [np.zeros_like(q) for _ in range(table.stages)]
+    ptrs = (_dp * table.stages)(*[_ptr(r) for r in res])
+    for _ in range(steps):
+        if lib().ora_rk_transform_step(C.byref(g), C.byref(gas), C.byref(flux), C.byref(bc or BC()), C.byref(table),
+                                       dt, _ptr(q), ptrs) != 0:
+            raise RuntimeError("ora_rk_transform_step failed")
+    return q
+
+
+def rk_identity_step(g, gas, flux, table, dt, q, steps=1):
+    """Explicit table with the identity transform on conserved storage (advance.h:107-230)."""
+    res = [np.zeros_like(q) for _ in range(table.stages)]
+    ptrs = (_dp * table.stages)(*[_ptr(r) for r in res])
+    q2, prim = np.zeros_like(q), np.zeros_like(q)
+    for _ in range(steps):
+        if lib().ora_rk_identity_step(C.byref(g), C.byref(gas), C.byref(flux), C.byref(table), dt, _ptr(q),
+                                      _ptr(q2), ptrs, _ptr(prim)) != 0:
+            raise RuntimeError("ora_rk_identity_step failed")
+    return q
 
 
 def boundary_fill_extrap(g, a, face_mask, order):

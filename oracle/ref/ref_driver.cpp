@@ -492,6 +492,116 @@ int main(int argc, char** argv)
             dump("g2_ssprk3opt_step3", time_int.solution());
         }
     }
+    //------------------------------------------------------------------
+    // Grid G2 again: the other Butcher tables of explicit.h.
+    //  * transform-aware advance (advance.h:233-280), primitive storage:
+    //    rk2_t, rk4_t, ssprk34_t, rk38r_t, 2 steps each;
+    //  * generic advance (advance.h:107-230) with the identity transform on
+    //    conserved storage (the residual callback converts to primitives):
+    //    rk4_t and rk38r_t (one solution copy), rk2hs_t and ssprk3hs_t (two),
+    //    2 steps each.
+    //------------------------------------------------------------------
+    {
+        spade::ctrs::array<int, 3> num_blocks(2, 2, 2);
+        spade::ctrs::array<int, 3> cells_in_block(8, 8, 8);
+        spade::ctrs::array<int, 3> nexch(2, 2, 2);
+        spade::bound_box_t<real_t, 3> bounds;
+        for (int d = 0; d < 3; ++d) { bounds.min(d) = 0.0; bounds.max(d) = twopi; }
+        spade::grid::cartesian_blocks_t blocks(num_blocks, bounds);
+        fix_boundary_flags(blocks);
+        spade::grid::cartesian_grid_t grid(cells_in_block, blocks, coords, group);
+        prim_t fill1 = 0.0;
+        flux_t fill2 = 0.0;
+        cons_t fill3 = 0.0;
+        spade::grid::grid_array prim0(grid, fill1, nexch, spade::device::cpu);
+        spade::grid::grid_array rhs (grid, fill2, nexch, spade::device::cpu);
+        spade::algs::fill_array(prim0, tgv);
+        auto handle = spade::grid::create_exchange(prim0, group, spade::ctrs::array<bool,3>(true, true, true));
+        handle.exchange(prim0);
+        power_law_all_t<real_t> visc_pow(mu_ref, T0, power, prandtl);
+        spade::viscous::visc_lr vsc_pow(visc_pow, air);
+        const auto flx = spade::omni::compose(spade::convective::cent_keep<2>(air), vsc_pow);
+        // dt as for the G2 runs above (CFL 0.5 at t = 0)
+        real_t umax = 0.0;
+        {
+            const auto img = prim0.image();
+            for (int lb = 0; lb < 8; ++lb)
+            for (int k = 0; k < 8; ++k) for (int j = 0; j < 8; ++j) for (int i = 0; i < 8; ++i)
+            {
+                const auto q = img.get_elem(spade::grid::cell_idx_t(i, j, k, lb));
+                const real_t s = std::sqrt(gamma*r_gas*q.T()) + std::sqrt(q.u()*q.u() + q.v()*q.v() + q.w()*q.w());
+                umax = std::max(umax, s);
+            }
+        }
+        const real_t dt = 0.5*grid.get_dx(0, 0)/umax;
+        auto calc_rhs = [&](auto& rhs_in, const auto& q, const auto& t) -> void
+        {
+            rhs_in = 0.0;
+            spade::pde_algs::flux_div(q, rhs_in, flx);
+        };
+        auto bc = [&](auto& q, const auto& t) { handle.exchange(q); };
+        cons_t transform_state;
+        spade::fluid_state::state_transform_t trans(transform_state, air);
+        auto run_trans = [&](const std::string& name, const auto& alg)
+        {
+            spade::grid::grid_array q = prim0;
+            spade::time_integration::time_axis_t axis(0.0, dt);
+            spade::time_integration::integrator_data_t data(q, rhs, alg);
+            spade::time_integration::integrator_t time_int(axis, alg, data, calc_rhs, bc, trans);
+            for (int n = 0; n < 2; ++n) time_int.advance();
+            dump("g2_" + name + "_step2", time_int.solution());
+        };
+        run_trans("rk2", spade::time_integration::rk2_t());
+        run_trans("rk4", spade::time_integration::rk4_t());
+        run_trans("ssprk34", spade::time_integration::ssprk34_t());
+        run_trans("rk38r", spade::time_integration::rk38r_t());
+
+        // conserved storage, identity transform
+        spade::grid::grid_array cons0(grid, fill3, nexch, spade::device::cpu);
+        spade::grid::grid_array pscr (grid, fill1, nexch, spade::device::cpu);
+        {
+            const auto pimg = prim0.image();
+            auto cimg = cons0.image();
+            for (int lb = 0; lb < 8; ++lb)
+            for (int k = -2; k < 10; ++k) for (int j = -2; j < 10; ++j) for (int i = -2; i < 10; ++i)
+            {
+                spade::grid::cell_idx_t idx(i, j, k, lb);
+                cons_t c;
+                spade::fluid_state::convert_state(pimg.get_elem(idx), c, air);
+                cimg.set_elem(idx, c);
+            }
+        }
+        auto cons_rhs = [&](auto& rhs_in, const auto& qc, const auto& t) -> void
+        {
+            const auto cimg = qc.image();
+            auto pimg = pscr.image();
+            for (int lb = 0; lb < 8; ++lb)
+            for (int k = -2; k < 10; ++k) for (int j = -2; j < 10; ++j) for (int i = -2; i < 10; ++i)
+            {
+                spade::grid::cell_idx_t idx(i, j, k, lb);
+                prim_t p;
+                spade::fluid_state::convert_state(cimg.get_elem(idx), p, air);
+                pimg.set_elem(idx, p);
+            }
+            rhs_in = 0.0;
+            spade::pde_algs::flux_div(pscr, rhs_in, flx);
+        };
+        auto cons_bc = [&](auto& qc, const auto& t) { handle.exchange(qc); };
+        auto run_ident = [&](const std::string& name, const auto& alg)
+        {
+            spade::grid::grid_array q = cons0;
+            spade::time_integration::time_axis_t axis(0.0, dt);
+            spade::time_integration::integrator_data_t data(q, rhs, alg);
+            spade::time_integration::integrator_t time_int(axis, alg, data, cons_rhs, cons_bc);
+            for (int n = 0; n < 2; ++n) time_int.advance();
+            dump("g2c_" + name + "_step2", time_int.solution(), true);
+        };
+        dump("g2c_cons0", cons0, true);
+        run_ident("rk4", spade::time_integration::rk4_t());
+        run_ident("rk38r", spade::time_integration::rk38r_t());
+        run_ident("rk2hs", spade::time_integration::rk2hs_t());
+        run_ident("ssprk3hs", spade::time_integration::ssprk3hs_t());
+    }
     g_manifest.close();
     return 0;
 }

@@ -1049,6 +1049,156 @@ int ora_ssprk3_step(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, c
     return 0;
 }
 
+// value of the reduced ratio n/d as the reference's coeff_value_t computes it
+// (numeric_t(num)/numeric_t(denum), time-integration/advance.h:33-46)
+static double ratio_value(long n, long d)
+{
+    long a = std::labs(n), b = std::labs(d);
+    while (b) { long t = a % b; a = b; b = t; }
+    if (a == 0) return 0.0;
+    n /= a; d /= a;
+    if (d < 0) { n = -n; d = -d; }
+    return double(n)/double(d);
+}
+
+// One step of an explicit table with the prim<->cons transform, advance.h:
+// 233-280: stage 0's residual from q; stage i advances q in place by the row
+// differences (a_i - a_{i-1}) dt (transform_advance_to, :56-102), then the
+// boundary and residual i; the last update takes (b - a_{s-1}) dt.  res holds
+// one array per stage.
+int ora_rk_transform_step(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, const ora_bc* bc,
+                          const ora_rk_table* tb, double dt, double* q, double* const* res)
+{
+    if (!g || !gas || !fl || !tb || !q || !res || tb->stages < 1 || tb->stages > ORA_RK_MAX) return -1;
+    const Geo G(g);
+    const int S = tb->stages;
+    auto row = [&](int r, int j, long& n, long& d)
+    {
+        if (r < S) { n = tb->a[r][j][0]; d = tb->a[r][j][1]; }
+        else       { n = tb->b[j][0];    d = tb->b[j][1]; }
+        if (d == 0) { n = 0; d = 1; }
+    };
+    auto advance_to = [&](int prev, int curr)
+    {
+        double tc[ORA_RK_MAX];
+        bool nz[ORA_RK_MAX];
+        for (int j = 0; j < S; ++j)
+        {
+            long n0, d0, n1, d1;
+            row(curr, j, n0, d0); row(prev, j, n1, d1);
+            const long num = d1*n0 - d0*n1, den = d0*d1;
+            nz[j] = num != 0;
+            tc[j] = nz[j] ? ratio_value(num, den)*dt : 0.0;
+        }
+        for_interior(G, [&](int i, int j, int k, int lb)
+        {
+            double* e = q + G.cell(i, j, k, lb);
+            double cons[5]; p2c(*gas, e, cons);
+            for (int st = 0; st < S; ++st)
+            {
+                if (!nz[st]) continue;
+                const double* rr = res[st] + G.cell(i, j, k, lb);
+                double inc[5];
+                for (int v = 0; v < 5; ++v) inc[v] = tc[st]*rr[v];
+                for (int v = 0; v < 5; ++v) cons[v] += inc[v];
+            }
+            c2p(*gas, cons, e);
+        });
+    };
+    flux_div(G, q, res[0], *fl, *gas, 0);
+    for (int st = 1; st < S; ++st)
+    {
+        advance_to(st - 1, st);
+        apply_bc(G, q, bc);
+        flux_div(G, q, res[st], *fl, *gas, 0);
+    }
+    advance_to(S - 1, S);
+    apply_bc(G, q, bc);
+    return 0;
+}
+
+// One step of an explicit table with the identity transform, advance.h:
+// 107-230, the state stored in conserved variables: the residual of a stage
+// converts every stored cell of the stage solution to primitives and takes
+// flux_div of them; the boundary is the exchange of the conserved array.
+// Residuals are scaled in place by dt a_ij, added to (and, with one solution
+// copy, afterwards subtracted from) the stage solution and scaled back by
+// 1/(dt a_ij), as the reference does.  q2: the second solution copy of a
+// high-storage table (unused otherwise); prim: scratch for the primitives.
+int ora_rk_identity_step(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, const ora_rk_table* tb,
+                         double dt, double* q, double* q2, double* const* res, double* prim)
+{
+    if (!g || !gas || !fl || !tb || !q || !res || !prim || tb->stages < 1 || tb->stages > ORA_RK_MAX) return -1;
+    if (tb->high_storage && !q2) return -1;
+    const Geo G(g);
+    const int S = tb->stages;
+    const size_t tot = (size_t)5*G.N[0]*G.N[1]*G.N[2]*G.nblk;
+    auto scale = [&](double* a, double c) { for (size_t n = 0; n < tot; ++n) a[n] *= c; };
+    auto incr = [&](double* a, const double* b, bool sub)
+    {
+        for_interior(G, [&](int i, int j, int k, int lb)
+        {
+            double* x = a + G.cell(i, j, k, lb);
+            const double* y = b + G.cell(i, j, k, lb);
+            for (int v = 0; v < 5; ++v) { if (sub) x[v] -= y[v]; else x[v] += y[v]; }
+        });
+    };
+    auto rhs = [&](double* r, const double* sol)
+    {
+        for (size_t c = 0; c < tot/5; ++c)
+        {
+            const double* cc = sol + 5*c;
+            double* pp = prim + 5*c;
+            // convert_state(cons -> prim), navier-stokes/fluid_state.h:118-135
+            const double rho = cc[0];
+            const double invrho = 1.0/rho;
+            const double u = invrho*cc[2], v = invrho*cc[3], w = invrho*cc[4];
+            const double rhoU2 = rho*(u*u + v*v + w*w);
+            const double p = (gas->gamma - 1.0)*(cc[1] - 0.5*rhoU2);
+            const double T = p/(gas->R*rho);
+            pp[0] = p; pp[1] = T; pp[2] = u; pp[3] = v; pp[4] = w;
+        }
+        flux_div(G, prim, r, *fl, *gas, 0);
+    };
+    auto a_val = [&](int i, int j) { return tb->a[i][j][1] == 0 ? 0.0 : ratio_value(tb->a[i][j][0], tb->a[i][j][1]); };
+    for (int i = 0; i < S; ++i)
+    {
+        double* sol = tb->high_storage ? q2 : q;
+        if (tb->high_storage) std::memcpy(q2, q, tot*sizeof(double));
+        for (int j = 0; j < i; ++j)
+        {
+            const double c = a_val(i, j);
+            if (c == 0.0) continue;
+            scale(res[j], dt*c);
+            incr(sol, res[j], false);
+            scale(res[j], 1.0/(dt*c));
+        }
+        if (i > 0) exchange(G, sol);
+        rhs(res[i], sol);
+        if (!tb->high_storage)
+        {
+            for (int j = 0; j < i; ++j)
+            {
+                const double c = a_val(i, j);
+                if (c == 0.0) continue;
+                scale(res[j], dt*c);
+                incr(sol, res[j], true);
+                scale(res[j], 1.0/(dt*c));
+            }
+        }
+    }
+    for (int i = 0; i < S; ++i)
+    {
+        const double c = tb->b[i][1] == 0 ? 0.0 : ratio_value(tb->b[i][0], tb->b[i][1]);
+        if (c == 0.0) continue;
+        scale(res[i], c*dt);
+        incr(q, res[i], false);
+        scale(res[i], 1.0/(c*dt));
+    }
+    exchange(G, q);
+    return 0;
+}
+
 // One low-storage step, ssprk3_opt (advance.h:285-402).
 int ora_ssprk3opt_step(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, const ora_bc* bc,
                        double dt, double* q, double* r0, double* r1)

@@ -82,6 +82,24 @@ int ora_ssprk3_step(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, c
 int ora_ssprk3opt_step(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, const ora_bc* bc,
                        double dt, double* q, double* r0, double* r1);
 
+/* explicit Butcher table (time-integration/explicit.h:27-116) as ratios
+ * {num, den}: a[i][j] (j < i), b[j], c[i]; a zero den means 0 */
+#define ORA_RK_MAX 6
+typedef struct
+{
+    int stages;
+    int high_storage;            /* two solution copies (var_size 2) */
+    long a[ORA_RK_MAX][ORA_RK_MAX][2];
+    long b[ORA_RK_MAX][2];
+    long c[ORA_RK_MAX][2];
+} ora_rk_table;
+/* one step, prim<->cons transform (advance.h:233-280); res: stages arrays */
+int ora_rk_transform_step(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, const ora_bc* bc,
+                          const ora_rk_table* tb, double dt, double* q, double* const* res);
+/* one step, identity transform on conserved storage (advance.h:107-230) */
+int ora_rk_identity_step(const ora_grid* g, const ora_gas* gas, const ora_flux* fl, const ora_rk_table* tb,
+                         double dt, double* q, double* q2, double* const* res, double* prim);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,7 +39,8 @@ __all__ = [
     "weno", "fweno", "weno_fds", "first_order", "visc_lr", "constant_viscosity", "power_law", "const_sensor",
     "ducros", "compose", "flux_div", "create_exchange", "boundary_fill_wall", "boundary_fill", "extrapolate",
     "FillKernel", "isothermal_wall", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "Integrator",
-    "ssprk3", "ssprk3_opt", "max_wavespeed", "tanh_map", "SpadeError", "diagnostics", "CaseSpec", "Batch",
+    "ssprk3", "ssprk3_opt", "RKIntegrator", "rk_table", "rk_stage", "rk2", "rk2hs", "rk4", "ssprk34",
+    "ssprk3_table", "ssprk3hs", "rk38r", "prim_cons", "identity_transform", "max_wavespeed", "tanh_map", "SpadeError", "diagnostics", "CaseSpec", "Batch",
     "grid_desc", "output_vtk", "output_vtk_host", "binary_write", "binary_read", "PRIM_NAMES", "CONS_NAMES",
     "FLUX_NAMES",
 ]
@@ -195,10 +196,38 @@ class GridArray:
         check(load().spd_array_copy(self._h, other._h))
         return self
 
+    # grid_array element-wise operations (grid/grid_array.h:286-396)
+    def __imul__(self, s):
+        check(load().spd_array_scale(self._h, float(s)))
+        return self
+
+    def __iadd__(self, other):
+        check(load().spd_array_add(self._h, other._h))
+        return self
+
+    def __isub__(self, other):
+        check(load().spd_array_sub(self._h, other._h))
+        return self
+
+    def convert(self, direction, gas, include_ghosts=True):
+        """convert_state in place: direction "cons" (prim -> cons) or "prim"."""
+        d = {"cons": _abi.TO_CONS, "prim": _abi.TO_PRIM}[direction]
+        check(load().spd_array_convert(self._h, d, gas.gamma, gas.R, int(bool(include_ghosts))))
+        return self
+
+    @classmethod
+    def _view(cls, grid, handle):
+        """A non-owning wrapper of an array handle the library passes to a callback."""
+        v = cls.__new__(cls)
+        v.grid = grid
+        v._h = C.c_void_p(handle)
+        v._view_only = True
+        return v
+
     def close(self):
-        if self._h:
+        if self._h and not getattr(self, "_view_only", False):
             load().spd_array_destroy(self._h)
-            self._h = C.c_void_p()
+        self._h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -461,6 +490,119 @@ class Integrator:
             self.close()
         except Exception:
             pass
+
+
+# explicit Butcher tables (time-integration/explicit.h:41-116)
+rk2, rk2hs, rk4, ssprk34, ssprk3_table, ssprk3hs, rk38r = (_abi.RK2, _abi.RK2HS, _abi.RK4, _abi.SSPRK34,
+                                                         _abi.SSPRK3_TABLE, _abi.SSPRK3HS, _abi.RK38R)
+# state transforms: state_transform_t(cons_t, gas) and identity_transform
+prim_cons, identity_transform = _abi.TRANS_PRIM_CONS, _abi.TRANS_IDENTITY
+
+
+def rk_table(which=None, a=None, b=None, c=None, high_storage=False):
+    """A built-in table (rk4, ...) or one given as lists of (num, den) ratios:
+    a[i][j] for j < i, b, c -- the rk_t<table, accum, dt> of explicit.h:27-38."""
+    t = _abi.RKTable()
+    if which is not None:
+        check(load().spd_rk_table_builtin(int(which), C.byref(t)))
+        return t
+    t.stages = len(b)
+    t.high_storage = int(bool(high_storage))
+    for i, row in enumerate(a):
+        for j, (n, d) in enumerate(row):
+            t.a[i][j].num, t.a[i][j].den = n, d
+    for i, (n, d) in enumerate(b):
+        t.b[i].num, t.b[i].den = n, d
+    for i, (n, d) in enumerate(c):
+        t.c[i].num, t.c[i].den = n, d
+    return t
+
+
+class RKIntegrator:
+    """integrator_t(axis, scheme, data, rhs, bc, trans) with any explicit table
+    (time-integration/integrator.h:18-59): trans = prim_cons (the solution holds
+    primitives; transform-aware advance, advance.h:233-280) or identity_transform
+    (generic advance on the stored variables, advance.h:107-230).  rhs(resid,
+    sol, t) and bc(sol, t) are Python callables on GridArray views; rhs = None is
+    flux_div(sol, resid, flux) (prim_cons only), bc = None the exchange plus the
+    isothermal walls."""
+
+    def __init__(self, q, table, dt, t0=0.0, trans=prim_cons, gas=None, flux=None, rhs=None, bc=None,
+                 walls=(), t_wall=0.0):
+        self.q = q
+        d = _abi.RKDesc()
+        d.table = rk_table(table) if isinstance(table, int) else table
+        d.trans = int(trans)
+        if gas is None and flux is not None:
+            gas = flux.terms()[0].gas
+        if gas is not None:
+            d.gamma, d.R = gas.gamma, gas.R
+        d.t0, d.dt = float(t0), float(dt)
+        self._errors = []
+        grid = q.grid
+
+        def guard(fn):
+            def call(*args):
+                try:
+                    fn(*args)
+                    return 0
+                except Exception as e:            # reported by advance()
+                    self._errors.append(e)
+                    return 1
+            return call
+        if rhs is not None:
+            self._rhs = _abi.RHS_FN(guard(lambda r, s, t, u: rhs(GridArray._view(grid, r), GridArray._view(grid, s), t)))
+            d.rhs = self._rhs
+        if bc is not None:
+            self._bc = _abi.BC_FN(guard(lambda s, t, u: bc(GridArray._view(grid, s), t)))
+            d.bc = self._bc
+        if flux is not None:
+            self._fd = flux_descriptor(flux)
+            d.flux = C.pointer(self._fd)
+        if walls:
+            self._walls = _abi.BCDesc()
+            for face in walls:
+                self._walls.wall_mask |= 1 << int(face)
+            self._walls.t_wall = float(t_wall)
+            d.walls = C.pointer(self._walls)
+        self._d = d
+        self._h = C.c_void_p()
+        check(load().spd_rk_integrator_create(q._h, C.byref(d), C.byref(self._h)))
+
+    def advance(self, nsteps=1):
+        rc = load().spd_integrator_advance(self._h, int(nsteps))
+        if self._errors:
+            e, self._errors = self._errors[0], []
+            raise e
+        check(rc)
+
+    def time(self):
+        t = C.c_double()
+        check(load().spd_integrator_time(self._h, C.byref(t)))
+        return t.value
+
+    def solution(self):
+        return self.q
+
+    def close(self):
+        if self._h:
+            load().spd_integrator_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def rk_stage(prims, resids, coefs, gas, push_ghosts=False):
+    """transform_advance_to (advance.h:56-102): cons = p2c(q); cons += coef_j r_j;
+    q = c2p(cons) on the interior cells; coefs are the caller's (a_ij - a_{i-1,j}) dt."""
+    n = len(resids)
+    hs = (C.c_void_p * n)(*[r._h.value for r in resids])
+    cs = (C.c_double * n)(*[float(c) for c in coefs])
+    check(load().spd_rk_stage(prims._h, hs, cs, n, gas.gamma, gas.R, int(bool(push_ghosts))))
 
 
 def diagnostics(prims, gas):

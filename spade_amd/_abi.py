@@ -64,6 +64,31 @@ class BCDesc(C.Structure):
 _vp = C.c_void_p
 _dp = C.POINTER(C.c_double)
 
+RK_MAX_STAGES = 6
+RK2, RK2HS, RK4, SSPRK34, SSPRK3_TABLE, SSPRK3HS, RK38R = range(7)
+TRANS_PRIM_CONS, TRANS_IDENTITY = 0, 1
+TO_CONS, TO_PRIM = 0, 1
+
+
+class Ratio(C.Structure):
+    _fields_ = [("num", C.c_long), ("den", C.c_long)]
+
+
+class RKTable(C.Structure):
+    _fields_ = [("stages", C.c_int), ("high_storage", C.c_int),
+                ("a", Ratio * RK_MAX_STAGES * RK_MAX_STAGES), ("b", Ratio * RK_MAX_STAGES),
+                ("c", Ratio * RK_MAX_STAGES)]
+
+
+RHS_FN = C.CFUNCTYPE(C.c_int, _vp, _vp, C.c_double, _vp)
+BC_FN = C.CFUNCTYPE(C.c_int, _vp, C.c_double, _vp)
+
+
+class RKDesc(C.Structure):
+    _fields_ = [("table", RKTable), ("trans", C.c_int), ("gamma", C.c_double), ("R", C.c_double),
+                ("t0", C.c_double), ("dt", C.c_double), ("rhs", RHS_FN), ("bc", BC_FN), ("user", _vp),
+                ("flux", C.POINTER(FluxDesc)), ("walls", C.POINTER(BCDesc))]
+
 # name -> (restype, argtypes); every function declared in include/spade_amd.h
 SIGNATURES = {
     "spd_last_error": (C.c_char_p, []),
@@ -94,6 +119,14 @@ SIGNATURES = {
     "spd_integrator_advance": (C.c_int, [_vp, C.c_int]),
     "spd_integrator_destroy": (C.c_int, [_vp]),
     "spd_integrator_residual": (_vp, [_vp, C.c_int]),
+    "spd_rk_table_builtin": (C.c_int, [C.c_int, C.POINTER(RKTable)]),
+    "spd_rk_integrator_create": (C.c_int, [_vp, C.POINTER(RKDesc), C.POINTER(_vp)]),
+    "spd_integrator_time": (C.c_int, [_vp, _dp]),
+    "spd_rk_stage": (C.c_int, [_vp, C.POINTER(_vp), _dp, C.c_int, C.c_double, C.c_double, C.c_int]),
+    "spd_array_scale": (C.c_int, [_vp, C.c_double]),
+    "spd_array_add": (C.c_int, [_vp, _vp]),
+    "spd_array_sub": (C.c_int, [_vp, _vp]),
+    "spd_array_convert": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int]),
     "spd_max_wavespeed": (C.c_int, [_vp, C.c_double, C.c_double, _dp]),
     "spd_diagnostics": (C.c_int, [_vp, C.c_double, _dp, _dp]),
     "spd_batch_create": (C.c_int, [_vp, C.c_longlong, C.c_int, C.POINTER(_vp)]),

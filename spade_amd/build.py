@@ -17,8 +17,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libspade_amd.so")
 OBJDIR = os.path.join(HERE, "_build")
 
-SOURCES = ["api.cpp", "io.cpp", "flux_generic.hip", "flux_fast.hip", "grid_ops.hip", "rk.hip", "batch.hip",
-           "batch_host.cpp"]
+SOURCES = ["api.cpp", "io.cpp", "rk_host.cpp", "flux_generic.hip", "flux_fast.hip", "grid_ops.hip", "rk.hip",
+           "batch.hip", "batch_host.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -16,17 +16,8 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 using namespace spd;
 
 struct spd_batch_s { std::unique_ptr<Batch> b; };
-struct spd_integrator_s
-{
-    spd_array_s* q = nullptr;
-    spd_flux_desc flux{};
-    spd_bc_desc bc{};
-    int scheme = 0;
-    double dt = 0;
-    spd_array_s* r[3] = {nullptr, nullptr, nullptr};
-};
 
-namespace {
+namespace spd {
 
 void validate_flux(const spd_flux_desc* f, int ng)
 {
@@ -138,7 +129,7 @@ void run_flux_div(const Grid& g, const double* prims, double* rhs, const spd_flu
         launch_flux_div_generic(g, prims, rhs, f, increment, g.ctx->stream);
 }
 
-} // namespace
+} // namespace spd
 
 extern "C" {
 
@@ -488,6 +479,11 @@ int spd_integrator_advance(spd_integrator_t it, int nsteps)
     {
         require(it != nullptr, "null integrator");
         require(nsteps >= 0, "nsteps must be >= 0");
+        if (it->generic)
+        {
+            for (int n = 0; n < nsteps; ++n) advance_generic(it);
+            return;
+        }
         Grid& g = *it->q->a.grid;
         double* q = it->q->a.d;
         const double dt = it->dt;
@@ -537,13 +533,17 @@ int spd_integrator_destroy(spd_integrator_t it)
     {
         if (!it) return;
         for (auto* r : it->r) free_array(r);
+        for (auto* r : it->res) free_array(r);
+        free_array(it->q2);
         delete it;
     });
 }
 
 spd_array_t spd_integrator_residual(spd_integrator_t it, int i)
 {
-    if (!it || i < 0 || i > 2) return nullptr;
+    if (!it) return nullptr;
+    if (it->generic) return (i >= 0 && i < it->table.stages) ? it->res[i] : nullptr;
+    if (i < 0 || i > 2) return nullptr;
     return it->r[i];
 }
 

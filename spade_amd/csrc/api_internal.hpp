@@ -6,6 +6,29 @@ struct spd_ctx_s   { spd::Ctx c; };
 struct spd_grid_s  { spd::Grid g; };
 struct spd_array_s { spd::Array a; };
 
+// An integrator: the fixed SSP-RK3 forms (scheme SPD_SCHEME_*, residuals r),
+// or a general explicit table (generic = true) with its callbacks
+struct spd_integrator_s
+{
+    spd_array_s* q = nullptr;
+    spd_flux_desc flux{};
+    spd_bc_desc bc{};
+    int scheme = 0;
+    double dt = 0;
+    spd_array_s* r[3] = {nullptr, nullptr, nullptr};
+    // general table
+    bool generic = false;
+    spd_rk_table table{};
+    int trans = SPD_TRANS_PRIM_CONS;
+    double gamma = 0, R = 0, t = 0;
+    spd_rhs_fn rhs = nullptr;
+    spd_bc_fn bcfn = nullptr;
+    void* user = nullptr;
+    bool has_flux = false, has_walls = false;
+    spd_array_s* res[SPD_RK_MAX_STAGES] = {};
+    spd_array_s* q2 = nullptr;          // second solution copy (high storage, identity transform)
+};
+
 namespace spd {
 
 // a file could not be opened, read or written
@@ -40,6 +63,17 @@ template <typename F> int guarded(F&& fn)
 
 // Checks a grid description (sizes, bounds, maps, ghost layers).
 void validate_grid_desc(const spd_grid_desc* d);
+// Checks a flux descriptor for a grid with ng ghost layers.
+void validate_flux(const spd_flux_desc* f, int ng);
+// Device arrays of a grid (zeroed) and their release (after the stream drains).
+spd_array_s* new_array(Grid& g);
+void free_array(spd_array_s* A);
+// flux_div with the kernel the library selects
+void run_flux_div(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f, bool increment);
+// The wall fills of the fixed integrators after a stage (the exchange is fused into the stage)
+void apply_bc(spd_integrator_s* it);
+// One step of a general-table integrator
+void advance_generic(spd_integrator_s* it);
 
 // Box minimum and cell size of block lb along axis a, as cartesian_blocks_t
 // and grid_geometry_t compute them (grid/cartesian_blocks.h:46-71,

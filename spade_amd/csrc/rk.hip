@@ -6,6 +6,8 @@
 // Compiled without contraction: the combination keeps the reference's
 // rounding sequence.
 #include "spd_internal.hpp"
+#include <algorithm>
+#include <string>
 
 namespace spd {
 namespace {
@@ -49,7 +51,8 @@ __device__ inline void c2p(double gamma, double R, const double c[5], double q[5
     q[0] = p; q[1] = T; q[2] = u; q[3] = v; q[4] = w;
 }
 
-struct Coefs { const double* r[3]; double c[3]; int n; };
+constexpr int kMaxStages = SPD_RK_MAX_STAGES;
+struct Coefs { const double* r[kMaxStages]; double c[kMaxStages]; int n; };
 
 // The exchange of the stage result, fused: interior cell (i, j, k) of block lb
 // is written into every ghost cell that holds it in another block.  Ghost
@@ -115,7 +118,7 @@ k_rk_transform(DevGeom G, double* __restrict__ q, Coefs C, double gamma, double 
     for (int v = 0; v < 5; ++v) { idx[v] = core_index(G, ci.lb, v, ci.i, ci.j, ci.k); qq[v] = q[idx[v]]; }
     p2c(gamma, R, qq, cons);
     #pragma unroll
-    for (int s = 0; s < 3; ++s)               // static indices: C stays in scalar registers
+    for (int s = 0; s < kMaxStages; ++s)      // static indices: C stays in scalar registers
     {
         if (s >= C.n || C.c[s] == 0.0) continue;
         double inc[5];
@@ -124,7 +127,7 @@ k_rk_transform(DevGeom G, double* __restrict__ q, Coefs C, double gamma, double 
     }
     c2p(gamma, R, cons, qq);
     for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
-    push_ghosts(G, push, q, ci, qq);
+    if (push) push_ghosts(G, push, q, ci, qq);
 }
 
 __global__ void __launch_bounds__(256)
@@ -177,6 +180,47 @@ k_rk_opt_s2(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, co
     push_ghosts(G, push, q, ci, qq);
 }
 
+// grid_array::operator*=(scalar) (grid/grid_array.h:369-379): every stored double
+__global__ void __launch_bounds__(256)
+k_array_scale(double* __restrict__ a, long long n, double s)
+{
+    for (long long e = blockIdx.x*(long long)blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x*blockDim.x)
+        a[e] *= s;
+}
+
+// grid_array::operator+= / -= (grid_array.h:286-318): a[v] += b[v] (or -=) on
+// the interior cells, variable by variable
+__global__ void __launch_bounds__(256)
+k_array_incr(DevGeom G, double* __restrict__ a, const double* __restrict__ b, int sub)
+{
+    CellIt ci;
+    if (!interior_cell(G, ci)) return;
+    for (int v = 0; v < 5; ++v)
+    {
+        const long long e = core_index(G, ci.lb, v, ci.i, ci.j, ci.k);
+        if (sub) a[e] -= b[e]; else a[e] += b[e];
+    }
+}
+
+// convert_state in place (navier-stokes/fluid_state.h:102-135) over every
+// stored cell of a block (ghosts = 1) or its interior cells
+__global__ void __launch_bounds__(256)
+k_state_convert(DevGeom G, double* __restrict__ a, int to_prim, double gamma, double R, int ghosts)
+{
+    const int ext0 = ghosts ? G.N[0] : G.nc[0], ext1 = ghosts ? G.N[1] : G.nc[1], ext2 = ghosts ? G.N[2] : G.nc[2];
+    const int c = blockIdx.x*blockDim.x + threadIdx.x;
+    if (c >= ext0*ext1*ext2) return;
+    const int off = ghosts ? G.ng : 0;
+    const int i = c % ext0 - off, j = (c / ext0) % ext1 - off, k = c / (ext0*ext1) - off;
+    const int lb = blockIdx.y;
+    long long idx[5];
+    double x[5], y[5];
+    for (int v = 0; v < 5; ++v) { idx[v] = cell_index(G, lb, v, i, j, k); x[v] = a[idx[v]]; }
+    if (to_prim) c2p(gamma, R, x, y);
+    else         p2c(gamma, R, x, y);
+    for (int v = 0; v < 5; ++v) a[idx[v]] = y[v];
+}
+
 inline dim3 interior_blocks(const DevGeom& G)
 {
     const int n = G.nc[0]*G.nc[1]*G.nc[2];
@@ -186,12 +230,37 @@ inline dim3 interior_blocks(const DevGeom& G)
 } // namespace
 
 void launch_rk_transform_stage(const Grid& g, double* q, const double* const* r, const double* coef, int nr,
-                               double gamma, double R, hipStream_t s)
+                               double gamma, double R, hipStream_t s, bool push)
 {
+    if (nr < 1 || nr > kMaxStages) throw ArgError{"a stage combines 1.." + std::to_string(kMaxStages) + " residuals"};
     Coefs C{};
     C.n = nr;
     for (int j = 0; j < nr; ++j) { C.r[j] = r[j]; C.c[j] = coef[j]; }
-    hipLaunchKernelGGL(k_rk_transform, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, C, gamma, R, g.d_push);
+    hipLaunchKernelGGL(k_rk_transform, interior_blocks(g.geom), dim3(256), 0, s, g.geom, q, C, gamma, R,
+                       push ? g.d_push : nullptr);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_array_scale(const Grid& g, double* a, double sc, hipStream_t s)
+{
+    const long long n = g.geom.sb*(long long)g.geom.nblk;
+    const unsigned wg = (unsigned)std::min<long long>((n + 255)/256, 8192);
+    hipLaunchKernelGGL(k_array_scale, dim3(wg), dim3(256), 0, s, a, n, sc);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_array_incr(const Grid& g, double* a, const double* b, bool sub, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_array_incr, interior_blocks(g.geom), dim3(256), 0, s, g.geom, a, b, sub ? 1 : 0);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_state_convert(const Grid& g, double* a, bool to_prim, double gamma, double R, bool ghosts, hipStream_t s)
+{
+    const DevGeom& G = g.geom;
+    const int n = ghosts ? G.N[0]*G.N[1]*G.N[2] : G.nc[0]*G.nc[1]*G.nc[2];
+    hipLaunchKernelGGL(k_state_convert, dim3((unsigned)((n + 255)/256), (unsigned)G.nblk), dim3(256), 0, s, G, a,
+                       to_prim ? 1 : 0, gamma, R, ghosts ? 1 : 0);
     SPD_LAUNCH_CHECK();
 }
 

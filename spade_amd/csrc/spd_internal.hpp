@@ -123,8 +123,14 @@ void launch_soa_to_aos(const Grid& g, const double* a, double* stage, int lb0, i
 // RK stage: q <- c2p(p2c(q) + sum_j coef[j] * r_j) on interior cells
 // The stage kernels also write every updated cell into the ghost cells of the
 // blocks it feeds (d_push): the exchange of the stage result, fused.
+// push = false leaves every ghost cell alone (interior update only)
 void launch_rk_transform_stage(const Grid& g, double* q, const double* const* r, const double* coef, int nr,
-                               double gamma, double R, hipStream_t s);
+                               double gamma, double R, hipStream_t s, bool push = true);
+// grid_array element-wise operations: scale every stored double; a (+|-)= b on
+// interior cells; convert_state in place (every stored cell or the interior)
+void launch_array_scale(const Grid& g, double* a, double sc, hipStream_t s);
+void launch_array_incr(const Grid& g, double* a, const double* b, bool sub, hipStream_t s);
+void launch_state_convert(const Grid& g, double* a, bool to_prim, double gamma, double R, bool ghosts, hipStream_t s);
 void launch_rk_opt_s0(const Grid& g, double* q, const double* r0, double dt, double gamma, double R, hipStream_t s);
 void launch_rk_opt_s1(const Grid& g, double* q, double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s);
 void launch_rk_opt_s2(const Grid& g, double* q, const double* r0, const double* r1, double dt, double gamma, double R, hipStream_t s);

@@ -58,6 +58,19 @@ int main(int argc, char** argv)
         sa::flux_div(q, rhs, flux);
         write(out + "/facade_rhs.bin", rhs.download());
         const double dt = 0.5*(L/32)/sa::max_wavespeed(q, air);
+        // rk4_t with the transform-aware advance and a caller residual that
+        // counts its calls around flux_div (integrator_t with a rhs callable)
+        {
+            sa::grid_array q4(g);
+            q4.upload(host.data());
+            sa::create_exchange(q4).exchange(q4);
+            int calls = 0;
+            sa::rk_integrator_t rk(q4, sa::rk_table(SPD_RK4), dt, sa::state_transform, air,
+                                   [&](sa::grid_array& r, const sa::grid_array& s, double) { ++calls; sa::flux_div(s, r, flux, false); });
+            rk.advance(2);
+            if (calls != 8) throw std::runtime_error("rk4 residual calls " + std::to_string(calls));
+            write(out + "/facade_rk4_step2.bin", q4.download());
+        }
         sa::integrator_t it(q, flux, dt, sa::ssprk3);
         it.advance(3);
         write(out + "/facade_step3.bin", q.download());

@@ -34,3 +34,27 @@ def test_symbol_exported(name):
 def test_python_signatures_cover_header():
     from spade_amd import _abi
     assert set(_declared()) <= set(_abi.SIGNATURES)
+
+
+def test_builtin_rk_tables_match_the_oracle():
+    """spd_rk_table_builtin (host-only call) gives the tables the oracle's golden
+    tests run (explicit.h:41-116)."""
+    from spade_amd import _abi
+    import spade_amd as sa
+    from oracle import pyoracle as po
+    sa.load()
+    names = {"rk2": sa.rk2, "rk2hs": sa.rk2hs, "rk4": sa.rk4, "ssprk34": sa.ssprk34, "ssprk3": sa.ssprk3_table,
+             "ssprk3hs": sa.ssprk3hs, "rk38r": sa.rk38r}
+    from fractions import Fraction as F
+
+    def frac(r):
+        return F(0) if r.den == 0 else F(r.num, r.den)
+    for name, which in names.items():
+        t = sa.rk_table(which)
+        a, b, c, hs = po.RK_TABLES[name]
+        assert t.stages == len(b) and bool(t.high_storage) == hs, name
+        for i in range(t.stages):
+            for j in range(_abi.RK_MAX_STAGES):
+                want = F(*a[i][j]) if j < len(a[i]) else F(0)
+                assert frac(t.a[i][j]) == want, (name, i, j)
+            assert frac(t.b[i]) == F(*b[i]) and frac(t.c[i]) == F(*c[i]), (name, i)

@@ -41,6 +41,13 @@ def test_facade_demo_matches_oracle():
     err = gi.rel_maxnorm(gi.interior(step3, 2), gi.interior(q, 2))
     print("facade 3 steps rel err", err)
     assert np.all(err <= 1e-12)
+    # rk_integrator_t with rk4_t, a caller residual, the transform-aware advance
+    rk4 = np.fromfile(os.path.join(d, "facade_rk4_step2.bin")).reshape(shape)
+    q = prim0.copy()
+    po.rk_transform_step(g, gas, fl, None, po.rk_table("rk4"), dt, q, steps=2)
+    err = gi.rel_maxnorm(gi.interior(rk4, 2), gi.interior(q, 2))
+    print("facade rk4 2 steps rel err", err)
+    assert np.all(err <= 1e-12)
     # io::binary_write wrote the host image; binary_read restored it; output_vtk wrote every block
     with open(os.path.join(d, "facade_step3.bin"), "rb") as a, \
             open(os.path.join(d, "facade_step3_checkpoint.bin"), "rb") as b, \

@@ -145,3 +145,31 @@ def test_golden_checksums():
             digest, name = line.split()
             with open(os.path.join(gi.GOLDEN, name), "rb") as f:
                 assert hashlib.sha256(f.read()).hexdigest() == digest, name
+
+
+@pytest.mark.parametrize("table", ["rk2", "rk4", "ssprk34", "rk38r"])
+def test_transform_aware_tables_bitwise(table):
+    """Two steps of the transform-aware advance (time-integration/advance.h:
+    233-280) with the other Butcher tables of explicit.h, primitive storage."""
+    g, q, fl = _g2_setup()
+    po.rk_transform_step(g, po.Gas(GAMMA, R), fl, None, po.rk_table(table), P["g2_dt"], q, steps=2)
+    assert np.array_equal(gi.interior(q, 2), gi.load(f"g2_{table}_step2"))
+
+
+def test_transform_aware_ssprk3_table_equals_fixed_form():
+    """The general-table path with the ssprk3_t table is the fixed ssprk3 step."""
+    g, q, fl = _g2_setup()
+    po.rk_transform_step(g, po.Gas(GAMMA, R), fl, None, po.rk_table("ssprk3"), P["g2_dt"], q, steps=1)
+    assert np.array_equal(gi.interior(q, 2), gi.load("g2_ssprk3_step1"))
+
+
+@pytest.mark.parametrize("table", ["rk4", "rk38r", "rk2hs", "ssprk3hs"])
+def test_identity_transform_tables_bitwise(table):
+    """Two steps of the generic advance (advance.h:107-230) with the identity
+    transform on conserved storage (residual: primitives of every stored cell,
+    then flux_div; boundary: exchange), one and two solution copies."""
+    g = _grid(G2)
+    q = gi.load("g2c_cons0").copy()
+    fl = _flux(FLUX_CASES["keep2_visc_pow"])
+    po.rk_identity_step(g, po.Gas(GAMMA, R), fl, po.rk_table(table), P["g2_dt"], q, steps=2)
+    assert np.array_equal(q, gi.load(f"g2c_{table}_step2"))
