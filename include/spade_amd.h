@@ -273,6 +273,16 @@ int spd_array_convert(spd_array_t arr, int direction, double gamma, double R, in
 /* ---- reductions on interior cells (transform_reduce, algs/transform_reduce.h:53-190) ---- */
 /* max(|u| + sqrt(gamma R T)): the CFL wave speed (development/cuda-tgv/main.cc:155-172) */
 int spd_max_wavespeed(spd_array_t prims, double gamma, double R, double* out);
+/* The same wave speed left in device memory, allocating nothing: no counterpart
+ * in the reference; for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops).  d_out: one double of device memory on the
+ * array's device; the call is stream-ordered on the context's stream and
+ * returns without waiting.  The per-block partials live in a static device
+ * array of 32768 doubles (256 KiB per device, held until the process ends):
+ * grids of more than 32768 blocks are refused (SPD_ERR_ARG).  All calls on one
+ * device share that array, so calls that overlap on one device (other streams,
+ * other threads) race; calls ordered on one stream do not. */
+int spd_max_wavespeed_noalloc(spd_array_t prims, double gamma, double R, double* d_out);
 
 /* Volume averages over interior cells of a primitive array with current ghosts:
  * kinetic energy <rho |u|^2 / 2> (rho = p/(R T)) and enstrophy <|omega|^2 / 2>,

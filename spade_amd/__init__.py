@@ -39,7 +39,7 @@ __all__ = [
     "weno", "fweno", "weno_fds", "first_order", "visc_lr", "constant_viscosity", "power_law", "const_sensor",
     "ducros", "compose", "flux_div", "create_exchange", "boundary_fill_wall", "boundary_fill", "extrapolate",
     "FillKernel", "isothermal_wall", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "Integrator",
-    "ssprk3", "ssprk3_opt", "RKIntegrator", "rk_table", "rk_stage", "rk2", "rk2hs", "rk4", "ssprk34",
+    "ssprk3", "ssprk3_opt", "max_wavespeed_noalloc", "RKIntegrator", "rk_table", "rk_stage", "rk2", "rk2hs", "rk4", "ssprk34",
     "ssprk3_table", "ssprk3hs", "rk38r", "prim_cons", "identity_transform", "max_wavespeed", "tanh_map", "SpadeError", "diagnostics", "CaseSpec", "Batch",
     "grid_desc", "output_vtk", "output_vtk_host", "binary_write", "binary_read", "PRIM_NAMES", "CONS_NAMES",
     "FLUX_NAMES",
@@ -723,6 +723,12 @@ def max_wavespeed(prims, gas):
     out = C.c_double()
     check(load().spd_max_wavespeed(prims._h, gas.gamma, gas.R, C.byref(out)))
     return out.value
+
+
+def max_wavespeed_noalloc(prims, gas, d_out):
+    """The CFL wave speed written to d_out (address of one double of device
+    memory on the array's device), stream-ordered, without allocating."""
+    check(load().spd_max_wavespeed_noalloc(prims._h, gas.gamma, gas.R, C.c_void_p(int(d_out))))
 
 
 # --------------------------------------------------------------------- io ---

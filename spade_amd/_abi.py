@@ -128,6 +128,7 @@ SIGNATURES = {
     "spd_array_sub": (C.c_int, [_vp, _vp]),
     "spd_array_convert": (C.c_int, [_vp, C.c_int, C.c_double, C.c_double, C.c_int]),
     "spd_max_wavespeed": (C.c_int, [_vp, C.c_double, C.c_double, _dp]),
+    "spd_max_wavespeed_noalloc": (C.c_int, [_vp, C.c_double, C.c_double, _vp]),
     "spd_diagnostics": (C.c_int, [_vp, C.c_double, _dp, _dp]),
     "spd_batch_create": (C.c_int, [_vp, C.c_longlong, C.c_int, C.POINTER(_vp)]),
     "spd_batch_destroy": (C.c_int, [_vp]),

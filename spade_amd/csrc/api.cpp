@@ -146,6 +146,7 @@ int spd_ctx_create(int device, spd_ctx_t* out)
         auto* c = new spd_ctx_s;
         c->c.device = device;
         SPD_HIP(hipSetDevice(device));
+        init_device_scratch(device);
         SPD_HIP(hipStreamCreateWithFlags(&c->c.stream, hipStreamNonBlocking));
         SPD_HIP(hipEventCreate(&c->c.ev0));
         SPD_HIP(hipEventCreate(&c->c.ev1));
@@ -554,6 +555,21 @@ int spd_max_wavespeed(spd_array_t prims, double gamma, double R, double* out)
         require(prims && out, "null argument");
         Grid& g = *prims->a.grid;
         *out = reduce_max_wavespeed(g, prims->a.d, gamma, R, g.ctx->stream);
+    });
+}
+
+int spd_max_wavespeed_noalloc(spd_array_t prims, double gamma, double R, double* d_out)
+{
+    return guarded([&]
+    {
+        require(prims && d_out, "null argument");
+        Grid& g = *prims->a.grid;
+        hipPointerAttribute_t attr{};
+        const hipError_t e = hipPointerGetAttributes(&attr, d_out);
+        if (e != hipSuccess) (void)hipGetLastError();     // a host pointer: clear the error it leaves
+        require(e == hipSuccess && attr.type == hipMemoryTypeDevice && attr.device == g.ctx->device,
+                "d_out must point to device memory of the array's device");
+        max_wavespeed_noalloc(g, prims->a.d, gamma, R, d_out, g.ctx->stream);
     });
 }
 

@@ -2,6 +2,8 @@
 #include "spd_internal.hpp"
 #include <cmath>
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace spd {
 namespace {
@@ -467,6 +469,45 @@ void launch_max_wavespeed(const DevGeom& G, const double* q, double gamma, doubl
 }
 
 } // namespace
+
+// Scratch of the allocation-free CFL reduction (spd_max_wavespeed_noalloc):
+// one partial per block, at most kScratchBlocks blocks (256 KiB per device,
+// held until the process ends).  Every call on one device shares that
+// device's copy and overwrites the partials it reads, so nothing needs
+// clearing; calls that overlap on one device race.  The array's name is a
+// host-side stand-in: host code uses the per-device address below, looked up
+// once per device at context creation with that device current.
+constexpr int kScratchBlocks = 32768;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kScratchBlocks];
+static double* g_scratch_ptr[kMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+void init_device_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        throw ArgError{"device index " + std::to_string(device) + " beyond " + std::to_string(kMaxDevices)};
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    void* ptr = nullptr;
+    SPD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[device] = static_cast<double*>(ptr);
+}
+
+void max_wavespeed_noalloc(Grid& g, const double* q, double gamma, double R, double* d_out, hipStream_t s)
+{
+    const DevGeom& G = g.geom;
+    if (G.nblk > kScratchBlocks)
+        throw ArgError{"the allocation-free reduction takes at most " + std::to_string(kScratchBlocks) +
+                       " blocks (grid has " + std::to_string(G.nblk) + ")"};
+    double* part = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratch_mu);
+        part = g_scratch_ptr[g.ctx->device];
+    }
+    if (!part) throw ArgError{"device scratch not set up for this context's device"};
+    launch_max_wavespeed(G, q, gamma, R, part, d_out, s);
+}
 
 double reduce_max_wavespeed(Grid& g, const double* q, double gamma, double R, hipStream_t s)
 {

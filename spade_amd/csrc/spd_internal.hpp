@@ -137,6 +137,11 @@ void launch_rk_opt_s2(const Grid& g, double* q, const double* r0, const double* 
 // both keep their reduction buffers with the grid (allocated on first use)
 void diagnostics(Grid& g, const double* q, double R, double& ke, double& en, hipStream_t s);
 double reduce_max_wavespeed(Grid& g, const double* q, double gamma, double R, hipStream_t s);
+// The CFL reduction with its partials in the device's static scratch, result
+// left in device memory d_out; allocates nothing.  init_device_scratch looks
+// up the scratch's address on `device` (which must be current), once.
+void init_device_scratch(int device);
+void max_wavespeed_noalloc(Grid& g, const double* q, double gamma, double R, double* d_out, hipStream_t s);
 
 // Device layout of one variable of one block ("core + x-halo"):
 //   core:   every stored row (j, k in [-ng, n+ng)), ghost rows included, holds
