@@ -321,6 +321,12 @@ int spd_batch_remove_case(spd_batch_t b, int case_id);
 /* Advance every active case by nsteps SSP-RK3 steps. */
 int spd_batch_step(spd_batch_t b, int nsteps);
 int spd_batch_case_info(spd_batch_t b, int case_id, long long* cells, int* steps_done, double* dt);
+/* Health of a case: *diverged_step = -1 while every recorded kinetic energy is
+ * finite, else the first step whose starting state had a non-finite one (the
+ * reference's driver stops on a NaN wave speed, development/cuda-tgv/main.cc:
+ * 250-259; here the case can be retired while the others go on: cases never
+ * read each other's blocks). */
+int spd_batch_case_status(spd_batch_t b, int case_id, int* diverged_step);
 /* Conserved state (rho, rhoE, rho u, rho v, rho w), [block][var][k][j][i], 16^3 per block. */
 int spd_batch_case_download(spd_batch_t b, int case_id, double* host);
 int spd_batch_case_upload(spd_batch_t b, int case_id, const double* host);

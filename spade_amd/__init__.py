@@ -678,6 +678,13 @@ class Batch:
         check(load().spd_batch_case_info(self._h, int(cid), C.byref(cells), C.byref(steps), C.byref(dt)))
         return {"cells": cells.value, "steps_done": steps.value, "dt": dt.value}
 
+    def status(self, cid):
+        """-1 while the case's kinetic energy stays finite, else the first step
+        whose starting state had a non-finite one."""
+        s = C.c_int()
+        check(load().spd_batch_case_status(self._h, int(cid), C.byref(s)))
+        return s.value
+
     def download(self, cid):
         n = self.info(cid)["cells"]
         out = np.empty(5 * n, dtype=np.float64)

@@ -137,6 +137,7 @@ SIGNATURES = {
     "spd_batch_remove_case": (C.c_int, [_vp, C.c_int]),
     "spd_batch_step": (C.c_int, [_vp, C.c_int]),
     "spd_batch_case_info": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_longlong), C.POINTER(C.c_int), _dp]),
+    "spd_batch_case_status": (C.c_int, [_vp, C.c_int, C.POINTER(C.c_int)]),
     "spd_batch_case_download": (C.c_int, [_vp, C.c_int, _dp]),
     "spd_batch_case_upload": (C.c_int, [_vp, C.c_int, _dp]),
     "spd_batch_case_diag": (C.c_int, [_vp, C.c_int, _dp, _dp, C.c_int, C.POINTER(C.c_int)]),

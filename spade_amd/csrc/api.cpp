@@ -642,6 +642,15 @@ int spd_batch_case_info(spd_batch_t b, int id, long long* cells, int* steps, dou
     });
 }
 
+int spd_batch_case_status(spd_batch_t b, int id, int* diverged_step)
+{
+    return guarded([&]
+    {
+        require(b && diverged_step, "null argument");
+        *diverged_step = b->b->get(id).diverged_step;
+    });
+}
+
 int spd_batch_case_download(spd_batch_t b, int id, double* host)
 {
     return guarded([&] { require(b && host, "null argument"); b->b->download(id, host); });

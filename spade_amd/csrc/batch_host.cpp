@@ -272,6 +272,7 @@ void Batch::collect_diag(int first_step, int nsteps)
             }
             c.ke.push_back(ke/ncell);
             c.en.push_back(en/ncell);
+            if (c.diverged_step < 0 && !std::isfinite(ke)) c.diverged_step = c.steps_done + t;
         }
         c.steps_done += nsteps;
     }

@@ -19,6 +19,7 @@ struct BatchCase
     std::vector<int> slots;          // slot of each block, block order x-fastest
     int steps_done = 0;
     std::vector<double> ke, en;      // per step: volume-averaged kinetic energy and enstrophy
+    int diverged_step = -1;          // first step whose starting state had a non-finite kinetic energy
 };
 
 struct TuneEntry { int grid; int interleaved; float ms_per_step; };

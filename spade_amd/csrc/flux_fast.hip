@@ -130,9 +130,9 @@ __device__ inline void pdiss_s(const FastCfg& C, const CellV& L, const CellV& R,
 
 // visc_lr row D for the unit normal of identity coordinates (viscous.h:39-112),
 // times the composition scale s (carried by mu, beta and the conductivity).
-template <int D, int LAW>
-__device__ inline void visc_s(const FastCfg& C, const FaceSums& f, const double g[3][3], double gT, double F[5],
-                              bool& bad)
+template <int D, int LAW, bool MAP>
+__device__ inline void visc_s(const FastCfg& C, const FaceSums& f, const double g[3][3], double gT, double nf,
+                              double F[5], bool& bad)
 {
     double mu, visc2, cond;
     if (LAW == SPD_LAW_POWER)
@@ -160,10 +160,21 @@ __device__ inline void visc_s(const FastCfg& C, const FaceSums& f, const double 
     const double hsum = 0.5*(f.SU*t[0] + f.SV*t[1] + f.SW*t[2]);
     const double ht = hsum + cond*gT;
     F[0] = 0.0;
-    F[1] = -ht;
-    F[2] = -t[0];
-    F[3] = -t[1];
-    F[4] = -t[2];
+    if constexpr (MAP)
+    {
+        // -(n . ht), -(n . tau) with the area-weighted face normal n = nf e_D
+        F[1] = -(nf*ht);
+        F[2] = -(nf*t[0]);
+        F[3] = -(nf*t[1]);
+        F[4] = -(nf*t[2]);
+    }
+    else
+    {
+        F[1] = -ht;
+        F[2] = -t[0];
+        F[3] = -t[1];
+        F[4] = -t[2];
+    }
 }
 
 template <int KEEP, int PDISS, int VISC> struct Spec
@@ -186,13 +197,19 @@ template <int KEEP, int PDISS, int VISC> struct Spec
 // Every factor is a power of two, so each rounding equals the reference's;
 // the other terms are evaluated pre-multiplied by 8 and the factor is folded
 // into 1/dx.
-template <int D, class S>
+// On mapped (diagonal) coordinates (MAP) the normals enter: mt = {n_D of the
+// left cell, n_D of the right cell, n_D of the face} (metric(), the area
+// factors of coord_system.h:250-267), u_n = u_D n_D (the other components of a
+// diagonal normal are zero); the result is then the flux scaled by the
+// composition scale, not yet by 1/dx: the cells apply their own
+// jac (1 - 2 pm)/dx (flux_div_basic.h:66-68).
+template <int D, class S, bool MAP>
 __device__ inline void face_product(const FastCfg& C, const CellV& L, const CellV& R, const double g[3][3], double gT,
-                                    double idx, double P[5], bool& bad)
+                                    double idx, const double mt[3], double P[5], bool& bad)
 {
     const FaceSums f = face_sums(L, R);
     double s8[5];
-    if (S::vs) visc_s<D, S::law>(C, f, g, gT, s8, bad);
+    if (S::vs) visc_s<D, S::law, MAP>(C, f, g, gT, mt[2], s8, bad);
     if (S::pd)
     {
         double z[5];
@@ -202,11 +219,27 @@ __device__ inline void face_product(const FastCfg& C, const CellV& L, const Cell
     }
     if (S::keep)
     {
-        const double unL = vel<D>(L), unR = vel<D>(R);
+        double unL = vel<D>(L), unR = vel<D>(R);
+        if constexpr (MAP) { unL = unL*mt[0]; unR = unR*mt[1]; }
         const double CX = (L.rho + R.rho)*(unR + unL);
         const double Ep = CX*f.K + CX*(L.e + R.e) + 4.0*(unL*R.p + unR*L.p);
         double M[3] = {CX*f.SU, CX*f.SV, CX*f.SW};
-        M[D] = M[D] + 4.0*f.SP;
+        if constexpr (MAP) M[D] = M[D] + 4.0*(f.SP*mt[2]);
+        else               M[D] = M[D] + 4.0*f.SP;
+        if constexpr (MAP)
+        {
+            // 8 F: continuity 2 CX (+ s8), the others as below; the cells scale by 1/8
+            if (S::vs || S::pd)
+            {
+                P[0] = S::pd ? 2.0*(CX + 0.5*s8[0]) : 2.0*CX;
+                P[1] = Ep + s8[1]; P[2] = M[0] + s8[2]; P[3] = M[1] + s8[3]; P[4] = M[2] + s8[4];
+            }
+            else
+            {
+                P[0] = 2.0*CX; P[1] = Ep; P[2] = M[0]; P[3] = M[1]; P[4] = M[2];
+            }
+            return;
+        }
         const double i4 = 0.25*idx, i8 = 0.125*idx;
         if (S::vs || S::pd)
         {
@@ -224,9 +257,27 @@ __device__ inline void face_product(const FastCfg& C, const CellV& L, const Cell
     else
     {
         #pragma unroll
-        for (int n = 0; n < 5; ++n) P[n] = s8[n]*idx;
+        for (int n = 0; n < 5; ++n) P[n] = MAP ? s8[n] : s8[n]*idx;
     }
 }
+
+// info::gradient on mapped coordinates (omni/infos/info_gradient.h, the
+// coordinate scaling of face_grad): component a of every gradient divided by
+// d x_a / d xi_a at the face point (mn for the face normal axis D, the cell
+// centre values mt1 / mt2 for the other two); identity axes divide by 1.
+template <int D>
+__device__ inline void grads_map(const DevGeom& G, double g[3][3], double& gT, double mD, double mA, double mB)
+{
+    constexpr int A = (D + 1) % 3, B = (D + 2) % 3;
+    if (G.map_kind[D] != SPD_MAP_IDENTITY) { for (int b = 0; b < 3; ++b) g[D][b] = g[D][b]/mD; gT = gT/mD; }
+    if (G.map_kind[A] != SPD_MAP_IDENTITY) for (int b = 0; b < 3; ++b) g[A][b] = g[A][b]/mA;
+    if (G.map_kind[B] != SPD_MAP_IDENTITY) for (int b = 0; b < 3; ++b) g[B][b] = g[B][b]/mB;
+}
+
+// metric() of a diagonal map at a point with derivatives m0, m1, m2: jac =
+// 1/(m0 m1 m2), normal component D = 1/(m_D jac) (core/coord_system.h:250-302)
+__device__ inline double jac_at(double m0, double m1, double m2) { return 1.0/(m0*m1*m2); }
+__device__ inline double nrm_at(double mD, double jac) { return 1.0/(mD*jac); }
 
 // 0.25-weighted four-point tangential gradient (info_gradient.h:63-79):
 // the reference sums r(0.25 q it) terms; 0.25 factors out exactly.
@@ -304,13 +355,17 @@ __device__ inline int lpos(int i, int j) { return (j + 1)*HX + (i + 1); }
 
 struct Raw { double p, T, u, v, w; };
 
-template <int KEEP, int PDISS, int VISC, bool INC>
+template <int KEEP, int PDISS, int VISC, bool INC, bool MAP>
 __global__ void __launch_bounds__(256, 2)
 k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, FastCfg C,
             int* __restrict__ start)
 {
     using S = Spec<KEEP, PDISS, VISC>;
     __shared__ int sBad[2];               // a face of plane k is outside the pow series range (by parity)
+    // mapped coordinates: d x_a / d xi_a of this block at cell centres -1 .. 16
+    // and at faces 0 .. 16 of each axis, from the same map_deriv as the
+    // per-cell kernels (so the same bits)
+    __shared__ double sMc[MAP ? 3 : 1][18], sMf[MAP ? 3 : 1][17];
     __shared__ double sU[3][3*HP];        // u, v, w of planes k-1, k, k+1 (ring)
     __shared__ double sS[2][4*HP];        // p, T, rho, e of planes k, k+1 (by parity)
     __shared__ double sFx[5][16*17];      // x-face products, faces i = 0..16 of row j
@@ -320,6 +375,19 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     const int tid = threadIdx.x;
     const int i = tid & 15, j = tid >> 4;
     const double ix = G.inv_dx[3*lb + 0], iy = G.inv_dx[3*lb + 1], iz = G.inv_dx[3*lb + 2];
+    if constexpr (MAP)
+    {
+        if (tid < 54)
+        {
+            const int a = tid/18, c = tid % 18 - 1;
+            sMc[a][c + 1] = map_deriv(G, a, G.bmin[3*lb + a] + (c + 0.5)*G.dx[3*lb + a]);
+        }
+        else if (tid < 105)
+        {
+            const int a = (tid - 54)/17, c = (tid - 54) % 17;
+            sMf[a][c] = map_deriv(G, a, G.bmin[3*lb + a] + ((c + 0.5) - 0.5)*G.dx[3*lb + a]);
+        }
+    }
     const int nz = G.nc[2];
     const long long sv = G.sv;
     const int N1 = G.N[1];
@@ -391,12 +459,23 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     }
     lds_barrier();
 
+    // mapped coordinates: this column's d x / d xi along x and y (cell centres)
+    const double m0i = MAP ? sMc[0][i + 1] : 1.0, m1j = MAP ? sMc[1][j + 1] : 1.0;
     double zlo[5];
     bool bad = false;                     // carried into plane 0's check
     {
-        double g[3][3], gT;
+        double g[3][3], gT, mt[3] = {1.0, 1.0, 1.0};
         grads_z<S>(ring[0], ring[1], pown, prev, cur0, ix, iy, iz, g, gT);
-        face_product<2, S>(C, prev, cur0, g, gT, iz, zlo, bad);
+        if constexpr (MAP)
+        {
+            // face -1/2: L = plane -1, R = plane 0
+            const double mL = sMc[2][0], mR = sMc[2][1], mf = sMf[2][0];
+            mt[0] = nrm_at(mL, jac_at(m0i, m1j, mL));
+            mt[1] = nrm_at(mR, jac_at(m0i, m1j, mR));
+            mt[2] = nrm_at(mf, jac_at(m0i, m1j, mf));
+            grads_map<2>(G, g, gT, mf, m0i, m1j);
+        }
+        face_product<2, S, MAP>(C, prev, cur0, g, gT, iz, mt, zlo, bad);
     }
 
     // One plane k: every face, the residual of its cells, and the entry of
@@ -412,6 +491,9 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
         const CellV cur = cell_from_lds(sk, s0, pown);
         const CellV nxt = cell_from_lds(sS[(k + 1) & 1], sp, pown);
         double zhi[5];
+        // mapped coordinates: plane k's d z / d zeta and this cell's Jacobian
+        const double m2k = MAP ? sMc[2][k + 1] : 1.0;
+        const double jc = MAP ? jac_at(m0i, m1j, m2k) : 1.0;
         // the three faces of all lanes, then the block-boundary x-faces
         // (wave 2) and y-faces (wave 3); the series pow has no branch, so the
         // three faces form one straight-line block
@@ -419,25 +501,49 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             // low x-face of (i,j): L = (i-1, j)
             const int pL = pown - 1;
             const CellV L = cell_from_lds(sk, s0, pL);
-            double g[3][3], gT, P[5];
+            double g[3][3], gT, P[5], mt[3] = {1.0, 1.0, 1.0};
             grads_x<S>(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
-            face_product<0, S>(C, L, cur, g, gT, ix, P, bad);
+            if constexpr (MAP)
+            {
+                const double mL = sMc[0][i], mf = sMf[0][i];
+                mt[0] = nrm_at(mL, jac_at(mL, m1j, m2k));
+                mt[1] = nrm_at(m0i, jc);
+                mt[2] = nrm_at(mf, jac_at(mf, m1j, m2k));
+                grads_map<0>(G, g, gT, mf, m1j, m2k);
+            }
+            face_product<0, S, MAP>(C, L, cur, g, gT, ix, mt, P, bad);
             for (int n = 0; n < 5; ++n) sFx[n][j*17 + i] = P[n];
         }
         {
             // low y-face of (i,j): L = (i, j-1)
             const int pL = pown - HX;
             const CellV L = cell_from_lds(sk, s0, pL);
-            double g[3][3], gT, P[5];
+            double g[3][3], gT, P[5], mt[3] = {1.0, 1.0, 1.0};
             grads_y<S>(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
-            face_product<1, S>(C, L, cur, g, gT, iy, P, bad);
+            if constexpr (MAP)
+            {
+                const double mL = sMc[1][j], mf = sMf[1][j];
+                mt[0] = nrm_at(mL, jac_at(m0i, mL, m2k));
+                mt[1] = nrm_at(m1j, jc);
+                mt[2] = nrm_at(mf, jac_at(m0i, mf, m2k));
+                grads_map<1>(G, g, gT, mf, m2k, m0i);
+            }
+            face_product<1, S, MAP>(C, L, cur, g, gT, iy, mt, P, bad);
             for (int n = 0; n < 5; ++n) sFy[n][j*16 + i] = P[n];
         }
         {
             // high z-face k+1/2: L = (i,j,k), R = (i,j,k+1)
-            double g[3][3], gT;
+            double g[3][3], gT, mt[3] = {1.0, 1.0, 1.0};
             grads_z<S>(s0, sp, pown, cur, nxt, ix, iy, iz, g, gT);
-            face_product<2, S>(C, cur, nxt, g, gT, iz, zhi, bad);
+            if constexpr (MAP)
+            {
+                const double mR = sMc[2][k + 2], mf = sMf[2][k + 1];
+                mt[0] = nrm_at(m2k, jc);
+                mt[1] = nrm_at(mR, jac_at(m0i, m1j, mR));
+                mt[2] = nrm_at(mf, jac_at(m0i, m1j, mf));
+                grads_map<2>(G, g, gT, mf, m0i, m1j);
+            }
+            face_product<2, S, MAP>(C, cur, nxt, g, gT, iz, mt, zhi, bad);
         }
         if (tid >= 128 && tid < 144)
         {
@@ -445,9 +551,17 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             const int jj = tid - 128;
             const int pL = lpos(15, jj), pR = lpos(16, jj);
             const CellV L = cell_from_lds(sk, s0, pL), Rr = cell_from_lds(sk, s0, pR);
-            double g[3][3], gT, P[5];
+            double g[3][3], gT, P[5], mt[3] = {1.0, 1.0, 1.0};
             grads_x<S>(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
-            face_product<0, S>(C, L, Rr, g, gT, ix, P, bad);
+            if constexpr (MAP)
+            {
+                const double m1 = sMc[1][jj + 1], mL = sMc[0][16], mR = sMc[0][17], mf = sMf[0][16];
+                mt[0] = nrm_at(mL, jac_at(mL, m1, m2k));
+                mt[1] = nrm_at(mR, jac_at(mR, m1, m2k));
+                mt[2] = nrm_at(mf, jac_at(mf, m1, m2k));
+                grads_map<0>(G, g, gT, mf, m1, m2k);
+            }
+            face_product<0, S, MAP>(C, L, Rr, g, gT, ix, mt, P, bad);
             for (int n = 0; n < 5; ++n) sFx[n][jj*17 + 16] = P[n];
         }
         if (tid >= 192 && tid < 208)
@@ -456,9 +570,17 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             const int ii = tid - 192;
             const int pL = lpos(ii, 15), pR = lpos(ii, 16);
             const CellV L = cell_from_lds(sk, s0, pL), Rr = cell_from_lds(sk, s0, pR);
-            double g[3][3], gT, P[5];
+            double g[3][3], gT, P[5], mt[3] = {1.0, 1.0, 1.0};
             grads_y<S>(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
-            face_product<1, S>(C, L, Rr, g, gT, iy, P, bad);
+            if constexpr (MAP)
+            {
+                const double m0 = sMc[0][ii + 1], mL = sMc[1][16], mR = sMc[1][17], mf = sMf[1][16];
+                mt[0] = nrm_at(mL, jac_at(m0, mL, m2k));
+                mt[1] = nrm_at(mR, jac_at(m0, mR, m2k));
+                mt[2] = nrm_at(mf, jac_at(m0, mf, m2k));
+                grads_map<1>(G, g, gT, mf, m2k, m0);
+            }
+            face_product<1, S, MAP>(C, L, Rr, g, gT, iy, mt, P, bad);
             for (int n = 0; n < 5; ++n) sFy[n][16*16 + ii] = P[n];
         }
         #pragma unroll
@@ -478,16 +600,37 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
         // rhs of cell (i,j,k): faces in the order x-, x+, y-, y+, z-, z+
         // (flux_div_basic.h:54-72; the high faces enter as -P)
         double* __restrict__ ro = Rown + k*plane;
+        // identity coordinates: the face products already carry 1/dx; mapped:
+        // each cell scales the (composition-scaled) face fluxes by its own
+        // jac/dx, i.e. s = jac (1 - 2 pm) inv_dx (flux_div_basic.h:66-68)
+        double fx = 1.0, fy = 1.0, fz = 1.0;
+        if constexpr (MAP)
+        {
+            const double sc = S::keep ? 0.125 : 1.0;
+            fx = sc*(jc*ix); fy = sc*(jc*iy); fz = sc*(jc*iz);
+        }
         #pragma unroll
         for (int n = 0; n < 5; ++n)
         {
             double r = rin[n];
-            r += sFx[n][j*17 + i];
-            r -= sFx[n][j*17 + i + 1];
-            r += sFy[n][j*16 + i];
-            r -= sFy[n][(j + 1)*16 + i];
-            r += zlo[n];
-            r -= zhi[n];
+            if constexpr (MAP)
+            {
+                r += sFx[n][j*17 + i]*fx;
+                r -= sFx[n][j*17 + i + 1]*fx;
+                r += sFy[n][j*16 + i]*fy;
+                r -= sFy[n][(j + 1)*16 + i]*fy;
+                r += zlo[n]*fz;
+                r -= zhi[n]*fz;
+            }
+            else
+            {
+                r += sFx[n][j*17 + i];
+                r -= sFx[n][j*17 + i + 1];
+                r += sFy[n][j*16 + i];
+                r -= sFy[n][(j + 1)*16 + i];
+                r += zlo[n];
+                r -= zhi[n];
+            }
             __builtin_nontemporal_store(r, ro + n*sv);    // streamed: nothing in this launch reads it again
             zlo[n] = zhi[n];
         }
@@ -529,11 +672,18 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
 template <int KEEP, int PDISS, int VISC>
 void launch_fast(const Grid& g, const double* prims, double* rhs, const FastCfg& C, bool increment, hipStream_t s)
 {
-    if (increment)
-        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, true>), dim3(g.geom.nblk), dim3(256), 0, s,
+    const bool map = !g.geom.identity;
+    if (increment && map)
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, true, true>), dim3(g.geom.nblk), dim3(256), 0, s,
+                           g.geom, prims, rhs, C, g.d_start);
+    else if (increment)
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, true, false>), dim3(g.geom.nblk), dim3(256), 0, s,
+                           g.geom, prims, rhs, C, g.d_start);
+    else if (map)
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, true>), dim3(g.geom.nblk), dim3(256), 0, s,
                            g.geom, prims, rhs, C, g.d_start);
     else
-        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false>), dim3(g.geom.nblk), dim3(256), 0, s,
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, false>), dim3(g.geom.nblk), dim3(256), 0, s,
                            g.geom, prims, rhs, C, g.d_start);
     SPD_LAUNCH_CHECK();
 }
@@ -557,7 +707,9 @@ namespace {
 // Selects the fast kernel for (grid, flux); false when the generic kernel must run.
 bool select_fast(const DevGeom& G, const spd_flux_desc& f, int& keep, int& pd, int& vs, FastCfg& C)
 {
-    if (!G.identity || G.nc[0] != 16 || G.nc[1] != 16 || G.ng < 1) return false;
+    // identity or diagonal (tanh) maps; a 16 x 16 cross-section
+    if (G.nc[0] != 16 || G.nc[1] != 16 || G.ng < 1) return false;
+    for (int a = 0; a < 3; ++a) if (G.map_kind[a] != SPD_MAP_IDENTITY && G.map_kind[a] != SPD_MAP_TANH) return false;
     if (const char* e = getenv("SPD_DISABLE_FAST")) if (e[0] == '1') return false;
     // terms must be a subsequence of [keep2, pdiss, visc] in that order
     keep = 0; pd = 0; vs = 0;

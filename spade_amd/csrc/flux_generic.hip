@@ -50,17 +50,6 @@ struct FaceIdx
     }
 };
 
-__device__ inline double map_deriv(const DevGeom& G, int d, double xi)
-{
-    if (G.map_kind[d] == SPD_MAP_TANH)
-    {
-        const double s = (xi - G.map_c[d])/G.map_h[d];
-        const double t = tanh(G.map_beta[d]*s);
-        return G.map_beta[d]*(1.0 - t*t)/G.map_tanhb[d];
-    }
-    return 1.0;
-}
-
 __device__ inline void metric(const DevGeom& G, int d, const double xi[3], double n[3])
 {
     if (G.identity) { n[0] = (d == 0); n[1] = (d == 1); n[2] = (d == 2); return; }

@@ -72,7 +72,7 @@ __device__ inline void push_one(const DevGeom& G, const int* __restrict__ push, 
 }
 
 __device__ inline void push_ghosts(const DevGeom& G, const int* __restrict__ push, double* __restrict__ q,
-                                   const CellIt& c, const double qq[5])
+                                   const CellIt& c, const double qq[5], bool skip_x = false)
 {
     const int ng = G.ng;
     const bool lo0 = c.i < ng, lo1 = c.j < ng, lo2 = c.k < ng;
@@ -80,8 +80,10 @@ __device__ inline void push_ghosts(const DevGeom& G, const int* __restrict__ pus
     if (!(lo0 | lo1 | lo2 | hi0 | hi1 | hi2)) return;
     if (G.nc[0] >= 2*ng && G.nc[1] >= 2*ng && G.nc[2] >= 2*ng)
     {
-        // at most one direction per axis: the cell's own face set, 1 to 7 regions
-        const int ex = lo0 ? 1 : (hi0 ? -1 : 0), ey = lo1 ? 1 : (hi1 ? -1 : 0), ez = lo2 ? 1 : (hi2 ? -1 : 0);
+        // at most one direction per axis: the cell's own face set, 1 to 7 regions;
+        // skip_x leaves every region across x to the staged plane flush
+        const int ex = (lo0 ? 1 : (hi0 ? -1 : 0))*(skip_x ? 0 : 1);
+        const int ey = lo1 ? 1 : (hi1 ? -1 : 0), ez = lo2 ? 1 : (hi2 ? -1 : 0);
         if (ex)             push_one(G, push, q, c, qq, ex, 0, 0);
         if (ey)             push_one(G, push, q, c, qq, 0, ey, 0);
         if (ez)             push_one(G, push, q, c, qq, 0, 0, ez);
@@ -108,9 +110,78 @@ __device__ inline void push_ghosts(const DevGeom& G, const int* __restrict__ pus
     }
 }
 
+// Regions across x written from LDS.  An x-ghost section holds one cell per
+// stored row, so a wave pushing its own cells writes 8-byte pieces spread over
+// a section; with the plane's cells within ng of an x face staged first, each
+// (section, variable) receives 16 consecutive rows from consecutive threads.
+// Needs a workgroup of 256 threads on one 16 x 16 plane and blocks at least
+// 2 ng wide (one direction per axis).  Slot s < ng holds cells i = s (fed to
+// ex = +1 at i + 16), slot ng + s holds i = 15 - s (ex = -1 at i - 16).
+struct XStage { double v[4][5][16]; };
+
+__device__ inline bool plane_staged(const DevGeom& G)
+{
+    return G.nc[0] == 16 && G.nc[1] == 16 && G.ng <= 2 && G.nc[2] >= 2*G.ng;
+}
+
+__device__ inline void stage_x(const DevGeom& G, XStage& xs, const CellIt& c, const double qq[5])
+{
+    int slot = -1;
+    if (c.i < G.ng) slot = c.i;
+    else if (c.i >= 16 - G.ng) slot = G.ng + (15 - c.i);
+    if (slot >= 0)
+        for (int v = 0; v < 5; ++v) xs.v[slot][v][c.j] = qq[v];
+}
+
+__device__ inline void flush_x(const DevGeom& G, const int* __restrict__ push, double* __restrict__ q,
+                               const XStage& xs, int lb, int k)
+{
+    const int ng = G.ng;
+    const int ez_k = k < ng ? 1 : (k >= G.nc[2] - ng ? -1 : 0);
+    const int n = 2*ng*5*16;
+    for (int t = threadIdx.x; t < n; t += blockDim.x)
+    {
+        const int slot = t/80, v = (t/16) % 5, j = t % 16;
+        const bool lo = slot < ng;
+        const int i = lo ? slot : 15 - (slot - ng);
+        const int ex = lo ? 1 : -1;
+        const int ey_j = j < ng ? 1 : (j >= 16 - ng ? -1 : 0);
+        const double val = xs.v[slot][v][j];
+        for (int a = 0; a < 2; ++a)
+        {
+            const int ey = a ? ey_j : 0;
+            if (a && !ey) continue;
+            for (int b = 0; b < 2; ++b)
+            {
+                const int ez = b ? ez_k : 0;
+                if (b && !ez) continue;
+                const int D = push[27*lb + (ex + 1) + 3*(ey + 1) + 9*(ez + 1)];
+                if (D < 0) continue;
+                q[cell_index(G, D, v, i + ex*16, j + ey*16, k + ez*G.nc[2])] = val;
+            }
+        }
+    }
+}
+
+// The stage result of this thread's cell into the ghosts it feeds: directly,
+// or, on staged planes, everything across x from LDS after a workgroup barrier
+// (every thread of the workgroup reaches it: a staged plane has no idle thread)
+__device__ inline void push_result(const DevGeom& G, const int* __restrict__ push, double* __restrict__ q,
+                                   const CellIt& c, const double qq[5], XStage& xs)
+{
+    if (!push) return;
+    const bool staged = plane_staged(G);
+    push_ghosts(G, push, q, c, qq, staged);
+    if (!staged) return;
+    stage_x(G, xs, c, qq);
+    lds_barrier();
+    flush_x(G, push, q, xs, c.lb, c.k);
+}
+
 __global__ void __launch_bounds__(256)
 k_rk_transform(DevGeom G, double* __restrict__ q, Coefs C, double gamma, double R, const int* __restrict__ push)
 {
+    __shared__ XStage xs;
     CellIt ci;
     if (!interior_cell(G, ci)) return;
     double qq[5], cons[5];
@@ -127,13 +198,14 @@ k_rk_transform(DevGeom G, double* __restrict__ q, Coefs C, double gamma, double 
     }
     c2p(gamma, R, cons, qq);
     for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
-    if (push) push_ghosts(G, push, q, ci, qq);
+    push_result(G, push, q, ci, qq, xs);
 }
 
 __global__ void __launch_bounds__(256)
 k_rk_opt_s0(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, double dt, double gamma, double R,
             const int* __restrict__ push)
 {
+    __shared__ XStage xs;
     CellIt ci;
     if (!interior_cell(G, ci)) return;
     double qq[5], c[5]; long long idx[5];
@@ -142,13 +214,14 @@ k_rk_opt_s0(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, do
     for (int v = 0; v < 5; ++v) c[v] += dt*r0[idx[v]];
     c2p(gamma, R, c, qq);
     for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
-    push_ghosts(G, push, q, ci, qq);
+    push_result(G, push, q, ci, qq, xs);
 }
 
 __global__ void __launch_bounds__(256)
 k_rk_opt_s1(DevGeom G, double* __restrict__ q, double* __restrict__ r0, const double* __restrict__ r1, double dt, double gamma, double R,
             const int* __restrict__ push)
 {
+    __shared__ XStage xs;
     CellIt ci;
     if (!interior_cell(G, ci)) return;
     double qq[5], c[5], old[5], nr[5]; long long idx[5];
@@ -160,13 +233,14 @@ k_rk_opt_s1(DevGeom G, double* __restrict__ q, double* __restrict__ r0, const do
     for (int v = 0; v < 5; ++v) c[v] -= dt*old[v];
     c2p(gamma, R, c, qq);
     for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
-    push_ghosts(G, push, q, ci, qq);
+    push_result(G, push, q, ci, qq, xs);
 }
 
 __global__ void __launch_bounds__(256)
 k_rk_opt_s2(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, const double* __restrict__ r1, double dt, double gamma, double R,
             const int* __restrict__ push)
 {
+    __shared__ XStage xs;
     CellIt ci;
     if (!interior_cell(G, ci)) return;
     double qq[5], c[5]; long long idx[5];
@@ -177,7 +251,7 @@ k_rk_opt_s2(DevGeom G, double* __restrict__ q, const double* __restrict__ r0, co
     for (int v = 0; v < 5; ++v) c[v] += two_thirds_dt*r1[idx[v]];
     c2p(gamma, R, c, qq);
     for (int v = 0; v < 5; ++v) q[idx[v]] = qq[v];
-    push_ghosts(G, push, q, ci, qq);
+    push_result(G, push, q, ci, qq, xs);
 }
 
 // grid_array::operator*=(scalar) (grid/grid_array.h:369-379): every stored double

@@ -162,6 +162,21 @@ __host__ __device__ inline long long stored_offset(const DevGeom& g, int i, int 
     return g.xh + (long long)section*g.N[1]*g.N[2] + row;
 }
 
+// d x / d xi of a diagonal map along axis d at computational coordinate xi
+// (core/coord_system.h:103-140): 1 for identity axes, for the tanh
+// stretching x = c + h tanh(beta s)/tanh(beta), s = (xi - c)/h:
+// beta (1 - tanh^2(beta s))/tanh(beta)
+__device__ inline double map_deriv(const DevGeom& G, int d, double xi)
+{
+    if (G.map_kind[d] == SPD_MAP_TANH)
+    {
+        const double s = (xi - G.map_c[d])/G.map_h[d];
+        const double t = tanh(G.map_beta[d]*s);
+        return G.map_beta[d]*(1.0 - t*t)/G.map_tanhb[d];
+    }
+    return 1.0;
+}
+
 // Index of a stored cell (i, j, k in [-ng, n+ng)) of variable v in block lb.
 __host__ __device__ inline long long cell_index(const DevGeom& g, int lb, int v, int i, int j, int k)
 {

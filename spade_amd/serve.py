@@ -15,7 +15,15 @@ Reynolds number and step count) arrive over time.  The loop
     rounds;
   * retires a case when it has done its step count: its final conserved
     fields and its per-step kinetic-energy / enstrophy history are copied back
-    and its slots are freed for the queue.
+    and its slots are freed for the queue;
+  * retires a case early, with an error, once its kinetic energy is no longer
+    finite (the engine's status(id); the reference's driver stops on a NaN
+    wave speed, development/cuda-tgv/main.cc:250-259): its slots go back to
+    the queue and the other cases go on, unaffected (cases never read each
+    other's blocks);
+  * stops backfilling once the head request has waited max_head_wait seconds:
+    freed slots are then kept for it, so a stream of small requests cannot
+    starve a large one.
 
 Per-case arithmetic does not depend on which other cases share the device
 (the work list only changes which blocks a kernel visits, never how a block
@@ -37,10 +45,11 @@ class Request:
 
 
 class Result:
-    def __init__(self, request, admitted, completed, state, ke, enstrophy):
+    def __init__(self, request, admitted, completed, state, ke, enstrophy, error=None):
         self.request = request
         self.admitted, self.completed = admitted, completed
         self.state, self.ke, self.enstrophy = state, ke, enstrophy
+        self.error = error          # None, or why the case was retired before its step count
 
     @property
     def latency(self):
@@ -60,10 +69,11 @@ def poisson_requests(n, rate, make_spec, seed=0):
 
 class Server:
     def __init__(self, engine, max_chunk=8, backfill=True, keep_state=True, clock=time.perf_counter,
-                 sleep=time.sleep):
+                 sleep=time.sleep, max_head_wait=2.0):
         self.engine = engine
         self.max_chunk = int(max_chunk)
         self.backfill = backfill
+        self.max_head_wait = float(max_head_wait)
         self.keep_state = keep_state
         self.clock, self.sleep = clock, sleep
         self.rounds = 0
@@ -97,15 +107,18 @@ class Server:
             self.engine.step(k)
             self.rounds += 1
             done_at = self.clock() - t0
+            status = getattr(self.engine, "status", None)
             for cid in list(active):
                 a = active[cid]
                 a[1] -= k
                 self.cell_updates += k * a[0].spec.num_cells
-                if a[1] == 0:
+                bad = status(cid) if status is not None else -1
+                if a[1] == 0 or bad >= 0:
                     state = self.engine.download(cid) if self.keep_state else None
                     ke, en = self.engine.diag(cid)
                     self.engine.remove(cid)
-                    results.append(Result(a[0], a[2], done_at, state, ke, en))
+                    err = None if bad < 0 else f"diverged: non-finite kinetic energy at step {bad}"
+                    results.append(Result(a[0], a[2], done_at, state, ke, en, err))
                     del active[cid]
         self.makespan = self.clock() - t0
         return results
@@ -119,7 +132,8 @@ class Server:
                 active[cid] = [req, req.spec.steps, now]
                 del queue[i]
                 continue
-            if not self.backfill:
+            # the head keeps the freed slots once it has waited too long
+            if not self.backfill or (i == 0 and now - req.arrival > self.max_head_wait):
                 break
             i += 1
 

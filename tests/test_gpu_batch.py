@@ -208,3 +208,43 @@ def test_served_stream_at_its_sizes_equals_alone_runs():
         assert np.array_equal(r.state, st), r.request.rid
         assert np.array_equal(r.ke, ke) and np.array_equal(r.enstrophy, en), r.request.rid
         print("request", r.request.rid, "served == alone", flush=True)
+
+
+class _PoisonBatch(sa.Batch):
+    """The device engine, with one NaN density planted in the cases whose
+    request is marked, right after admission (a case that has diverged)."""
+
+    def __init__(self, budget, poisoned):
+        super().__init__(budget)
+        self.poisoned = poisoned
+
+    def add(self, spec):
+        cid = super().add(spec)
+        if id(spec) in self.poisoned:
+            st = self.download(cid)
+            st[0, 0, 3, 4, 5] = np.nan
+            self.upload(cid, st)
+        return cid
+
+
+def test_served_case_that_diverges_is_retired_others_unaffected():
+    """A served case whose state is no longer finite is retired with an error
+    after the round in which its kinetic energy turned non-finite; the other
+    cases, sharing the device with it, still equal their alone runs bit for
+    bit, and its slots return to the pool."""
+    specs = [_tgv(32, 12, mach=0.1), _tgv(32, 12, mach=0.2), sa.CaseSpec("vortex", (32, 32, 16), mach=0.3,
+                                                                          reynolds=400.0, steps=12)]
+    reqs = [Request(i, s, arrival=0.002 * i) for i, s in enumerate(specs)]
+    eng = _PoisonBatch(64 * MB, {id(specs[1])})
+    srv = Server(eng, max_chunk=3)
+    res = srv.run(reqs)
+    by = {r.request.rid: r for r in res}
+    assert len(res) == 3
+    assert by[1].error is not None and "diverged" in by[1].error, by[1].error
+    assert len(by[1].ke) < 12 and not np.isfinite(by[1].ke[-1])
+    for rid in (0, 2):
+        assert by[rid].error is None
+        st, (ke, en) = _run_alone(specs[rid], specs[rid].steps)
+        assert np.array_equal(by[rid].state, st), rid
+        assert np.array_equal(by[rid].ke, ke), rid
+    assert eng.fits(specs[0]) and eng.fits(specs[1])

@@ -108,3 +108,47 @@ def test_poisson_arrivals_and_summary():
     res = srv.run(reqs)
     s = summarize(res, srv.makespan, srv.cell_updates)
     assert s["requests"] == 50 and s["p99_latency_s"] >= s["p50_latency_s"] > 0
+
+
+class DivergingEngine(FakeEngine):
+    """A fake engine whose case `bad_cid` has a non-finite kinetic energy from step `at` on."""
+
+    def __init__(self, blocks, bad_cid, at):
+        super().__init__(blocks)
+        self.bad_cid, self.at = bad_cid, at
+
+    def status(self, cid):
+        return self.at if cid == self.bad_cid and self.cases[cid][1] > self.at else -1
+
+
+def test_diverged_case_retired_with_error_others_complete():
+    eng = DivergingEngine(blocks=64, bad_cid=1, at=3)
+    clk = FakeClock()
+    reqs = [Request(i, CaseSpec(cells=(32, 32, 32), steps=20), arrival=0.0) for i in range(2)]
+    reqs.append(Request(2, CaseSpec(cells=(32, 32, 32), steps=20), arrival=0.05))   # keeps rounds short
+    res = Server(eng, max_chunk=2, clock=clk, sleep=clk.sleep).run(reqs)
+    by = {r.request.rid: r for r in res}
+    assert by[1].error is not None and "step 3" in by[1].error
+    assert by[1].state[0] < 20                         # retired before its step count
+    for rid in (0, 2):
+        assert by[rid].error is None and by[rid].state[0] == 20
+    assert eng.free == 64 and not eng.cases
+
+
+def test_head_request_not_starved_by_backfill():
+    """A steady stream of small requests keeps a pool of 64 blocks partly busy;
+    a request for the whole pool at the head of the queue is admitted once it
+    has waited max_head_wait: backfill stops and the freed slots wait for it."""
+    eng = FakeEngine(blocks=64)
+    clk = FakeClock()
+    small = CaseSpec(cells=(32, 32, 32), steps=6)       # 8 blocks
+    big = CaseSpec(cells=(64, 64, 64), steps=6)         # 64 blocks
+    reqs = [Request(0, small, 0.0), Request(1, big, 0.0005)]
+    reqs += [Request(2 + i, small, 0.001 + 0.002 * i) for i in range(200)]
+    srv = Server(eng, max_chunk=2, clock=clk, sleep=clk.sleep, max_head_wait=0.05)
+    res = srv.run(reqs)
+    by = {r.request.rid: r for r in res}
+    assert len(res) == 202
+    later = [r for r in res if r.request.rid >= 2 and r.admitted > by[1].admitted]
+    assert later, "the big request must be admitted while small ones still arrive"
+    assert by[1].admitted - by[1].request.arrival < 0.05 + 0.05
