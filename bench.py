@@ -371,6 +371,17 @@ def main():
     g.ctx.sync()
     print(f"setup {time.perf_counter() - t0:.1f}s, {g.num_cells} cells", file=sys.stderr, flush=True)
 
+    # the path's other rows first (they change q: RK steps, wall ghosts), then
+    # the headline on the analytic state again; the GPU has then left its idle
+    # clocks, so the timed launches measure the kernel's steady state
+    rows = None
+    if not args.no_rows:
+        rows = rows_measure(sa, g, q, exch, gas, flux)
+        rows["channel_flux_div"] = channel_measure(sa)
+        q.upload(host)
+        exch.exchange(q)
+        g.ctx.sync()
+
     for _ in range(args.warmup):
         sa.flux_div(q, rhs, flux, increment=False)
     g.ctx.sync()
@@ -422,11 +433,9 @@ def main():
         g.ctx.sync()
         # the runtime copies the whole device allocation
         line["copy_check_bytes"] = 8 * g.device_size
-    if not args.no_rows:
-        line["rows"] = rows_measure(sa, g, q, exch, gas, flux)
+    if rows is not None:
+        line["rows"] = rows
     del q, rhs
-    if not args.no_rows:
-        line["rows"]["channel_flux_div"] = channel_measure(sa)
     if not args.no_chosen_form:
         line["chosen_form"] = chosen_form(args, log=lambda m: print(m, file=sys.stderr, flush=True))
     if not args.no_cpu_baseline:
