@@ -714,6 +714,18 @@ class Batch:
     def set_shape(self, grid, interleaved=False):
         check(load().spd_batch_set_shape(self._h, int(grid), int(bool(interleaved))))
 
+    def set_kernel_shape(self, prefetch=1, occupancy=3):
+        """Stage kernel shape: planes of state in flight (1, 2) and workgroups
+        per CU its registers are sized for (3, 2)."""
+        check(load().spd_batch_set_kernel_shape(self._h, int(prefetch), int(occupancy)))
+
+    def shape(self):
+        """The launch shape in use: grid, interleaved, prefetch, occupancy."""
+        v = [C.c_int() for _ in range(4)]
+        check(load().spd_batch_get_shape(self._h, *[C.byref(x) for x in v]))
+        return {"grid": v[0].value, "interleaved": bool(v[1].value), "prefetch": v[2].value,
+                "occupancy": v[3].value}
+
     def close(self):
         if self._h:
             load().spd_batch_destroy(self._h)

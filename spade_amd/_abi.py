@@ -144,6 +144,8 @@ SIGNATURES = {
     "spd_batch_tune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "spd_batch_use_graph": (C.c_int, [_vp, C.c_int]),
     "spd_batch_set_shape": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "spd_batch_set_kernel_shape": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "spd_batch_get_shape": (C.c_int, [_vp] + [C.POINTER(C.c_int)]*4),
     "spd_io_output_vtk": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p)]),
     "spd_io_output_vtk_host": (C.c_int, [C.POINTER(GridDesc), _dp, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p)]),
     "spd_io_binary_write": (C.c_int, [_vp, C.c_char_p]),

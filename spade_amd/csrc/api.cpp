@@ -699,6 +699,29 @@ int spd_batch_set_shape(spd_batch_t b, int grid, int interleaved)
     });
 }
 
+int spd_batch_set_kernel_shape(spd_batch_t b, int prefetch, int occupancy)
+{
+    return guarded([&]
+    {
+        require(b != nullptr, "null argument");
+        require(prefetch == 1 || prefetch == 2, "prefetch must be 1 or 2");
+        require(occupancy == 2 || occupancy == 3, "occupancy must be 2 or 3");
+        b->b->variant = (prefetch - 1) | (occupancy == 2 ? 2 : 0);
+    });
+}
+
+int spd_batch_get_shape(spd_batch_t b, int* grid, int* interleaved, int* prefetch, int* occupancy)
+{
+    return guarded([&]
+    {
+        require(b && grid && interleaved && prefetch && occupancy, "null argument");
+        *grid = b->b->grid;
+        *interleaved = b->b->order_interleaved ? 1 : 0;
+        *prefetch = spd::batch_variant_prefetch(b->b->variant);
+        *occupancy = spd::batch_variant_occupancy(b->b->variant);
+    });
+}
+
 } // extern "C"
 
 namespace spd {

@@ -28,11 +28,6 @@ namespace {
 
 constexpr int HXB = 18, HPB = 18*18;
 
-// planes of conserved state in flight ahead of the one entering the LDS window
-#ifndef SPD_BATCH_PREFETCH
-#define SPD_BATCH_PREFETCH 1
-#endif
-constexpr int PFB = SPD_BATCH_PREFETCH;
 
 struct CellF { float p, T, u, v, w, rho, e; };   // p, T, rho, e are perturbations
 
@@ -145,10 +140,12 @@ __device__ inline CellF to_f32(const CaseParams& P, const PrimD& q)
                  (float)(q.rho - P.rho_ref), (float)(P.cv*Tp)};
 }
 
-// Three workgroups per CU (168 VGPRs, 52.5 KB LDS): measured faster than two
-// despite a few spilled registers.
-template <int LAW, bool HAS_U0>
-__global__ void __launch_bounds__(256, 3)
+// Launch shapes the autotuner chooses between: PFB planes of conserved state
+// in flight ahead of the one entering the LDS window (1 or 2), and OCC
+// workgroups per CU the register budget is sized for (3: 168 VGPRs with a few
+// spills, 2: 256 VGPRs without; LDS is 52.5 KB either way).
+template <int LAW, bool HAS_U0, int PFB, int OCC>
+__global__ void __launch_bounds__(256, OCC)
 k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q0, double* __restrict__ Qout,
         double alpha, double beta, int want_diag)
 {
@@ -451,31 +448,61 @@ k_init_case(const int* __restrict__ stage, int nslots, CaseParams P, CaseIC ic, 
 
 } // namespace
 
-void launch_batch_stage(const BatchDev& B, int law, const double* Qin, const double* Q0, double* Qout,
+namespace {
+
+// shape variant v: PFB = 1 + (v & 1), OCC = (v & 2) ? 2 : 3
+template <int LAW, bool HAS_U0>
+void launch_shape(int v, const BatchDev& B, const double* Qin, const double* Q0, double* Qout,
+                  double alpha, double beta, int want_diag, int grid, hipStream_t s)
+{
+    switch (v)
+    {
+    case 0:  hipLaunchKernelGGL((k_stage<LAW, HAS_U0, 1, 3>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
+    case 1:  hipLaunchKernelGGL((k_stage<LAW, HAS_U0, 2, 3>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
+    case 2:  hipLaunchKernelGGL((k_stage<LAW, HAS_U0, 1, 2>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
+    default: hipLaunchKernelGGL((k_stage<LAW, HAS_U0, 2, 2>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
+    }
+}
+
+template <int LAW>
+int occupancy(int v)
+{
+    int n = 0;
+    switch (v)
+    {
+    case 0:  SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<LAW, true, 1, 3>, 256, 0)); break;
+    case 1:  SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<LAW, true, 2, 3>, 256, 0)); break;
+    case 2:  SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<LAW, true, 1, 2>, 256, 0)); break;
+    default: SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<LAW, true, 2, 2>, 256, 0)); break;
+    }
+    return n;
+}
+
+} // namespace
+
+void launch_batch_stage(const BatchDev& B, int law, int variant, const double* Qin, const double* Q0, double* Qout,
                         double alpha, double beta, int want_diag, int grid, hipStream_t s)
 {
+    if (variant < 0 || variant >= kBatchShapeVariants) throw ArgError{"batch kernel shape variant out of range"};
     // stage 1 of the Shu-Osher form (alpha = 0) reads no u0
     const bool u0 = alpha != 0.0;
     if (law == SPD_LAW_POWER)
     {
-        if (u0) hipLaunchKernelGGL((k_stage<SPD_LAW_POWER, true>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag);
-        else    hipLaunchKernelGGL((k_stage<SPD_LAW_POWER, false>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag);
+        if (u0) launch_shape<SPD_LAW_POWER, true>(variant, B, Qin, Q0, Qout, alpha, beta, want_diag, grid, s);
+        else    launch_shape<SPD_LAW_POWER, false>(variant, B, Qin, Q0, Qout, alpha, beta, want_diag, grid, s);
     }
     else
     {
-        if (u0) hipLaunchKernelGGL((k_stage<SPD_LAW_CONST, true>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag);
-        else    hipLaunchKernelGGL((k_stage<SPD_LAW_CONST, false>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag);
+        if (u0) launch_shape<SPD_LAW_CONST, true>(variant, B, Qin, Q0, Qout, alpha, beta, want_diag, grid, s);
+        else    launch_shape<SPD_LAW_CONST, false>(variant, B, Qin, Q0, Qout, alpha, beta, want_diag, grid, s);
     }
     SPD_LAUNCH_CHECK();
 }
 
-int batch_stage_blocks_per_cu(int law)
+int batch_stage_blocks_per_cu(int law, int variant)
 {
-    int n = 0;
-    if (law == SPD_LAW_POWER)
-        SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SPD_LAW_POWER, true>, 256, 0));
-    else
-        SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<SPD_LAW_CONST, true>, 256, 0));
+    if (variant < 0 || variant >= kBatchShapeVariants) throw ArgError{"batch kernel shape variant out of range"};
+    const int n = law == SPD_LAW_POWER ? occupancy<SPD_LAW_POWER>(variant) : occupancy<SPD_LAW_CONST>(variant);
     return std::max(1, n);
 }
 

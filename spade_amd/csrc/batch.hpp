@@ -39,12 +39,19 @@ struct BatchDev
     int hist_len;
 };
 
-void launch_batch_stage(const BatchDev& B, int law, const double* Qin, const double* Q0, double* Qout,
+// Kernel shape variants of the stage kernel: variant v keeps 1 + (v & 1)
+// planes of state in flight and sizes registers for (v & 2) ? 2 : 3
+// workgroups per CU.
+constexpr int kBatchShapeVariants = 4;
+inline int batch_variant_prefetch(int v) { return 1 + (v & 1); }
+inline int batch_variant_occupancy(int v) { return (v & 2) ? 2 : 3; }
+
+void launch_batch_stage(const BatchDev& B, int law, int variant, const double* Qin, const double* Q0, double* Qout,
                         double alpha, double beta, int want_diag, int grid, hipStream_t s);
 void launch_batch_step_count(const BatchDev& B, hipStream_t s);
 // Workgroups of the stage kernel (for law) resident per CU at once: the
 // default persistent grid is this many per CU
-int batch_stage_blocks_per_cu(int law);
+int batch_stage_blocks_per_cu(int law, int variant);
 // d_stage: per block {slot, case table, 27 neighbour slots}; also fills B.nbr and B.slot_case
 void launch_batch_init(const BatchDev& B, const int* d_stage, int nslots, const CaseParams& P, const CaseIC& ic,
                        double* Q, hipStream_t s);

@@ -51,7 +51,7 @@ Batch::Batch(Ctx* c, long long budget_bytes, int law_in, int hist) : ctx(c), law
     SPD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
     num_cu = std::max(1, cus);
     // as many persistent workgroups as fit at once (spd_batch_tune may pick another)
-    grid = batch_stage_blocks_per_cu(law)*num_cu;
+    grid = batch_stage_blocks_per_cu(law, variant)*num_cu;
     dev = BatchDev{d_worklist, d_small + 0, d_slot_case, d_nbr, d_cases, d_diag, d_small + 1, max_slots, hist_len};
 }
 
@@ -221,9 +221,9 @@ void Batch::enqueue_step(hipStream_t s)
 {
     // Shu-Osher SSP-RK3: u1 = u0 + dt L(u0); u2 = 3/4 u0 + 1/4 (u1 + dt L(u1));
     // u3 = 1/3 u0 + 2/3 (u2 + dt L(u2)).  Q0 -> Q1 -> Q2 -> Q0.
-    launch_batch_stage(dev, law, Q[0], Q[0], Q[1], 0.0, 1.0, 1, grid, s);
-    launch_batch_stage(dev, law, Q[1], Q[0], Q[2], 0.75, 0.25, 0, grid, s);
-    launch_batch_stage(dev, law, Q[2], Q[0], Q[0], 1.0/3.0, 2.0/3.0, 0, grid, s);
+    launch_batch_stage(dev, law, variant, Q[0], Q[0], Q[1], 0.0, 1.0, 1, grid, s);
+    launch_batch_stage(dev, law, variant, Q[1], Q[0], Q[2], 0.75, 0.25, 0, grid, s);
+    launch_batch_stage(dev, law, variant, Q[2], Q[0], Q[0], 1.0/3.0, 2.0/3.0, 0, grid, s);
     launch_batch_step_count(dev, s);
 }
 
@@ -237,6 +237,7 @@ void Batch::build_graph()
     SPD_HIP(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0));
     SPD_HIP(hipGraphDestroy(g));
     graph_grid = grid;
+    graph_variant = variant;
 }
 
 void Batch::collect_diag(int first_step, int nsteps)
@@ -286,7 +287,7 @@ void Batch::step(int nsteps)
     while (done < nsteps)
     {
         const int chunk = std::min(nsteps - done, hist_len);
-        if (use_graph && (!gexec || graph_grid != grid)) build_graph();
+        if (use_graph && (!gexec || graph_grid != grid || graph_variant != variant)) build_graph();
         for (int t = 0; t < chunk; ++t)
         {
             if (use_graph) SPD_HIP(hipGraphLaunch(gexec, ctx->stream));
@@ -337,10 +338,12 @@ void Batch::upload(int id, const double* host)
     SPD_HIP(hipStreamSynchronize(ctx->stream));
 }
 
-// Autotuner: times one step for each candidate launch shape (persistent grid
-// of 1..4 workgroups per CU, case-major or interleaved work order) on the
-// current batch and keeps the fastest.  State is saved and restored around
-// the trials so tuning does not advance any case.
+// Autotuner: times one step for each candidate launch shape on the current
+// batch and keeps the fastest.  Candidates: the stage kernel's shape variant
+// (1 or 2 planes of state in flight; registers sized for 3 or 2 workgroups per
+// CU), the persistent grid (1..4 workgroups per CU) and the work order
+// (case-major or interleaved).  State is saved and restored around the trials
+// so tuning does not advance any case.
 int Batch::tune()
 {
     if (worklist_dirty) upload_worklist();
@@ -356,27 +359,32 @@ int Batch::tune()
     int saved_step = 0;
     SPD_HIP(hipMemcpyAsync(&saved_step, d_small + 1, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     SPD_HIP(hipStreamSynchronize(ctx->stream));
-    float best = 1e30f; int best_grid = grid; bool best_order = order_interleaved;
+    float best = 1e30f; int best_grid = grid, best_variant = variant; bool best_order = order_interleaved;
     tune_log.clear();
     for (int order = 0; order < 2; ++order)
     {
         order_interleaved = (order == 1);
         upload_worklist();
-        for (int per_cu = 1; per_cu <= 4; ++per_cu)
+        for (int v = 0; v < kBatchShapeVariants; ++v)
         {
-            grid = per_cu*num_cu;
-            enqueue_step(ctx->stream);                  // warm-up
-            SPD_HIP(hipEventRecord(ctx->ev0, ctx->stream));
-            for (int r = 0; r < 2; ++r) enqueue_step(ctx->stream);
-            SPD_HIP(hipEventRecord(ctx->ev1, ctx->stream));
-            SPD_HIP(hipEventSynchronize(ctx->ev1));
-            float ms = 0.0f;
-            SPD_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
-            tune_log.push_back({grid, order, ms/2.0f});
-            if (ms < best) { best = ms; best_grid = grid; best_order = order_interleaved; }
+            variant = v;
+            for (int per_cu = 1; per_cu <= 4; ++per_cu)
+            {
+                grid = per_cu*num_cu;
+                enqueue_step(ctx->stream);              // warm-up
+                SPD_HIP(hipEventRecord(ctx->ev0, ctx->stream));
+                for (int r = 0; r < 2; ++r) enqueue_step(ctx->stream);
+                SPD_HIP(hipEventRecord(ctx->ev1, ctx->stream));
+                SPD_HIP(hipEventSynchronize(ctx->ev1));
+                float ms = 0.0f;
+                SPD_HIP(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+                tune_log.push_back({grid, order, v, ms/2.0f});
+                if (ms < best) { best = ms; best_grid = grid; best_order = order_interleaved; best_variant = v; }
+            }
         }
     }
     grid = best_grid;
+    variant = best_variant;
     order_interleaved = best_order;
     upload_worklist();
     for_slot_runs(saved_slots, [&](size_t b, size_t e) {

@@ -22,7 +22,7 @@ struct BatchCase
     int diverged_step = -1;          // first step whose starting state had a non-finite kinetic energy
 };
 
-struct TuneEntry { int grid; int interleaved; float ms_per_step; };
+struct TuneEntry { int grid; int interleaved; int variant; float ms_per_step; };
 
 struct Batch
 {
@@ -32,6 +32,7 @@ struct Batch
     int max_slots = 0, max_cases = 0;
     int num_cu = 1;
     int grid = 0, graph_grid = -1;
+    int variant = 0, graph_variant = -1;   // stage kernel shape (batch.hpp)
     bool use_graph = true;
     bool order_interleaved = false;
     bool worklist_dirty = true;

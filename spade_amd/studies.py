@@ -104,7 +104,9 @@ def run_ensemble(specs, steps, tune=True):
     blocks = sum(s.blocks for s in specs)
     b = Batch(int(blocks * BYTES_PER_BLOCK * 1.02) + (64 << 20))
     ids = [b.add(s) for s in specs]
-    shape = b.tune() if tune else None
+    if tune:
+        b.tune()
+    shape = b.shape()
     b.step(1)
     b.ctx.sync()
     t0 = time.perf_counter()

@@ -6,7 +6,8 @@
   stated bar for this form is 1e-5 on kinetic energy);
 * periodic conservation of mass and total energy to round-off;
 * a case's result is bit-for-bit the same alone and in a mix, with or without
-  the hipGraph, and after the autotuner ran;
+  the hipGraph, after the autotuner ran, and with every stage kernel shape
+  (prefetch depth, register budget) the autotuner chooses between;
 * admission fails cleanly when a case does not fit the HBM budget.
 """
 import numpy as np
@@ -111,6 +112,34 @@ def test_alone_equals_mix_bitwise():
     ke, en = b.diag(cid)
     assert np.array_equal(st, ref_state)
     assert np.array_equal(ke, ref_ke) and np.array_equal(en, ref_en)
+
+
+@pytest.mark.parametrize("prefetch,occupancy", [(1, 3), (2, 3), (1, 2), (2, 2)])
+def test_kernel_shapes_agree_bitwise(prefetch, occupancy):
+    spec = sa.CaseSpec("vortex", (32, 16, 32), mach=0.3, reynolds=700.0)
+    ref_state, (ref_ke, _) = _run_alone(spec, 6)
+    b = sa.Batch(256 * MB)
+    b.set_kernel_shape(prefetch, occupancy)
+    b.set_shape(768, interleaved=True)
+    cid = b.add(spec)
+    b.add(_tgv(16, 4))
+    b.step(6)
+    sh = b.shape()
+    assert (sh["prefetch"], sh["occupancy"]) == (prefetch, occupancy), sh
+    assert np.array_equal(b.download(cid), ref_state)
+    assert np.array_equal(b.diag(cid)[0], ref_ke)
+
+
+def test_tuner_reports_a_candidate_shape():
+    b = sa.Batch(256 * MB)
+    b.add(_tgv(32, 2))
+    b.add(sa.CaseSpec("vortex", (32, 16, 16), mach=0.2, reynolds=900.0))
+    g, inter = b.tune()
+    sh = b.shape()
+    assert sh["grid"] == g and sh["interleaved"] == inter
+    assert sh["prefetch"] in (1, 2) and sh["occupancy"] in (2, 3)
+    with pytest.raises(sa.SpadeError):
+        b.set_kernel_shape(3, 3)
 
 
 def test_graph_and_direct_launch_agree_bitwise():
