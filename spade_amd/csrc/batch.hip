@@ -143,6 +143,31 @@ __device__ inline CellF to_f32(const CaseParams& P, const PrimD& q)
                  (float)(q.rho - P.rho_ref), (float)(P.cv*Tp)};
 }
 
+__device__ inline double uniform_d(double x)
+{
+    const long long b = __double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(b & 0xffffffffll));
+    const int hi = __builtin_amdgcn_readfirstlane((int)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | lo));
+}
+__device__ inline float uniform_f(float x) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(x))); }
+
+// a wave-uniform copy of a case's constants (every lane reads the same entry)
+__device__ inline CaseParams uniform_params(const CaseParams& c)
+{
+    CaseParams u;
+    u.dt = uniform_d(c.dt); u.ix = uniform_d(c.ix); u.iy = uniform_d(c.iy); u.iz = uniform_d(c.iz);
+    u.gamma = uniform_d(c.gamma); u.gm1 = uniform_d(c.gm1); u.R = uniform_d(c.R); u.inv_R = uniform_d(c.inv_R);
+    u.cv = uniform_d(c.cv);
+    u.rho_ref = uniform_d(c.rho_ref); u.p_ref = uniform_d(c.p_ref); u.T_ref = uniform_d(c.T_ref);
+    u.e_ref = uniform_d(c.e_ref);
+    u.f_rho_ref = uniform_f(c.f_rho_ref); u.f_T_ref = uniform_f(c.f_T_ref); u.f_mu_ref = uniform_f(c.f_mu_ref);
+    u.f_inv_T_law = uniform_f(c.f_inv_T_law); u.f_power = uniform_f(c.f_power);
+    u.f_cp_over_pr = uniform_f(c.f_cp_over_pr);
+    u.law = __builtin_amdgcn_readfirstlane(c.law);
+    return u;
+}
+
 // Launch shapes the autotuner chooses between: PFB planes of conserved state
 // in flight ahead of the one entering the LDS window (1 or 2), and OCC
 // workgroups per CU the register budget is sized for (3: 168 VGPRs with a few
@@ -173,8 +198,10 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
 
     for (int w = blockIdx.x; w < nwork; w += gridDim.x)
     {
-        const int slot = B.worklist[w];
-        const CaseParams& P = B.cases[B.slot_case[slot]];
+        // slot and case constants are the same for the whole workgroup: held
+        // in scalar registers, not reloaded per lane inside the k-march
+        const int slot = __builtin_amdgcn_readfirstlane(B.worklist[w]);
+        const CaseParams P = uniform_params(B.cases[__builtin_amdgcn_readfirstlane(B.slot_case[slot])]);
         const int* nb = B.nbr + 27*slot;
         const float ix = (float)P.ix, iy = (float)P.iy, iz = (float)P.iz;
         // the slots this thread's two columns (own, halo) read from below the
