@@ -334,21 +334,35 @@ int spd_batch_case_upload(spd_batch_t b, int case_id, const double* host);
  * state at the start of each step; *n = number of entries written (<= max). */
 int spd_batch_case_diag(spd_batch_t b, int case_id, double* ke, double* enstrophy, int max, int* n);
 /* Autotuner: times the launch shapes on the current batch (state is restored)
- * and keeps the fastest: stage kernel shape (planes of state in flight 1|2,
- * registers sized for 3|2 workgroups per CU), persistent workgroups (1..4 per
+ * and keeps the fastest: stage kernel shape (rows of a block's 16x16
+ * cross-section per workgroup 16|8, planes of state in flight 1|2, registers
+ * sized for 3|2 waves per SIMD), persistent workgroups (1..4 blocks' worth per
  * CU) and work order.  Returns the chosen persistent grid and work order; the
- * kernel shape is read with spd_batch_get_shape. */
+ * kernel shape is read with spd_batch_get_shape.  Every shape gives the same
+ * bits. */
 int spd_batch_tune(spd_batch_t b, int* grid, int* interleaved);
 /* 1: replay each step from a hipGraph (default); 0: launch the kernels directly. */
 int spd_batch_use_graph(spd_batch_t b, int enable);
 /* Launch shape in use (persistent workgroups, work order). */
 int spd_batch_set_shape(spd_batch_t b, int grid, int interleaved);
-/* Stage kernel shape in use: prefetch = planes of state in flight ahead of the
- * LDS window (1 or 2), occupancy = workgroups per CU its registers are sized
- * for (3 or 2).  Other values are an argument error. */
-int spd_batch_set_kernel_shape(spd_batch_t b, int prefetch, int occupancy);
+/* Stage kernel shape in use: rows = rows of a block's 16x16 cross-section per
+ * workgroup (16: one 256-thread workgroup per block, 8: two 128-thread ones),
+ * prefetch = planes of state in flight ahead of the LDS window (1 or 2),
+ * occupancy = waves per SIMD its registers are sized for (3 or 2).  Other
+ * values are an argument error.  The persistent grid (spd_batch_set_shape)
+ * counts workgroups of the chosen size. */
+int spd_batch_set_kernel_shape(spd_batch_t b, int rows, int prefetch, int occupancy);
 /* The launch shape in use (after spd_batch_tune, the chosen one). */
-int spd_batch_get_shape(spd_batch_t b, int* grid, int* interleaved, int* prefetch, int* occupancy);
+int spd_batch_get_shape(spd_batch_t b, int* grid, int* interleaved, int* rows, int* prefetch, int* occupancy);
+/* One candidate the last spd_batch_tune timed. */
+typedef struct
+{
+    int grid, interleaved, rows, prefetch, occupancy;
+    float ms_per_step;
+} spd_tune_entry;
+/* The candidates of the last spd_batch_tune in the order they were timed;
+ * *n = number written (<= max). */
+int spd_batch_tune_log(spd_batch_t b, spd_tune_entry* out, int max, int* n);
 
 /* ---- on-disk formats: spade::io (io/io_vtk.h, io/io_native.h) ---- */
 

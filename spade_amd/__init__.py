@@ -714,17 +714,26 @@ class Batch:
     def set_shape(self, grid, interleaved=False):
         check(load().spd_batch_set_shape(self._h, int(grid), int(bool(interleaved))))
 
-    def set_kernel_shape(self, prefetch=1, occupancy=3):
-        """Stage kernel shape: planes of state in flight (1, 2) and workgroups
-        per CU its registers are sized for (3, 2)."""
-        check(load().spd_batch_set_kernel_shape(self._h, int(prefetch), int(occupancy)))
+    def set_kernel_shape(self, rows=16, prefetch=1, occupancy=3):
+        """Stage kernel shape: rows of a block's cross-section per workgroup
+        (16, 8), planes of state in flight (1, 2) and waves per SIMD its
+        registers are sized for (3, 2)."""
+        check(load().spd_batch_set_kernel_shape(self._h, int(rows), int(prefetch), int(occupancy)))
+
+    def tune_log(self):
+        """Candidates timed by the last tune(): list of dicts, in timing order."""
+        buf = (_abi.TuneEntry * 256)()
+        n = C.c_int()
+        check(load().spd_batch_tune_log(self._h, C.cast(buf, C.c_void_p), 256, C.byref(n)))
+        return [{"grid": e.grid, "interleaved": bool(e.interleaved), "rows": e.rows, "prefetch": e.prefetch,
+                 "occupancy": e.occupancy, "ms_per_step": e.ms_per_step} for e in buf[:n.value]]
 
     def shape(self):
-        """The launch shape in use: grid, interleaved, prefetch, occupancy."""
-        v = [C.c_int() for _ in range(4)]
+        """The launch shape in use: grid, interleaved, rows, prefetch, occupancy."""
+        v = [C.c_int() for _ in range(5)]
         check(load().spd_batch_get_shape(self._h, *[C.byref(x) for x in v]))
-        return {"grid": v[0].value, "interleaved": bool(v[1].value), "prefetch": v[2].value,
-                "occupancy": v[3].value}
+        return {"grid": v[0].value, "interleaved": bool(v[1].value), "rows": v[2].value,
+                "prefetch": v[3].value, "occupancy": v[4].value}
 
     def close(self):
         if self._h:

@@ -74,6 +74,11 @@ class Ratio(C.Structure):
     _fields_ = [("num", C.c_long), ("den", C.c_long)]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("grid", C.c_int), ("interleaved", C.c_int), ("rows", C.c_int), ("prefetch", C.c_int),
+                ("occupancy", C.c_int), ("ms_per_step", C.c_float)]
+
+
 class RKTable(C.Structure):
     _fields_ = [("stages", C.c_int), ("high_storage", C.c_int),
                 ("a", Ratio * RK_MAX_STAGES * RK_MAX_STAGES), ("b", Ratio * RK_MAX_STAGES),
@@ -144,8 +149,9 @@ SIGNATURES = {
     "spd_batch_tune": (C.c_int, [_vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "spd_batch_use_graph": (C.c_int, [_vp, C.c_int]),
     "spd_batch_set_shape": (C.c_int, [_vp, C.c_int, C.c_int]),
-    "spd_batch_set_kernel_shape": (C.c_int, [_vp, C.c_int, C.c_int]),
-    "spd_batch_get_shape": (C.c_int, [_vp] + [C.POINTER(C.c_int)]*4),
+    "spd_batch_set_kernel_shape": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int]),
+    "spd_batch_get_shape": (C.c_int, [_vp] + [C.POINTER(C.c_int)]*5),
+    "spd_batch_tune_log": (C.c_int, [_vp, C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "spd_io_output_vtk": (C.c_int, [_vp, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p)]),
     "spd_io_output_vtk_host": (C.c_int, [C.POINTER(GridDesc), _dp, C.c_char_p, C.c_char_p, C.POINTER(C.c_char_p)]),
     "spd_io_binary_write": (C.c_int, [_vp, C.c_char_p]),

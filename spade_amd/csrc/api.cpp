@@ -699,24 +699,44 @@ int spd_batch_set_shape(spd_batch_t b, int grid, int interleaved)
     });
 }
 
-int spd_batch_set_kernel_shape(spd_batch_t b, int prefetch, int occupancy)
+int spd_batch_set_kernel_shape(spd_batch_t b, int rows, int prefetch, int occupancy)
 {
     return guarded([&]
     {
         require(b != nullptr, "null argument");
+        require(rows == 16 || rows == 8, "rows must be 16 or 8");
         require(prefetch == 1 || prefetch == 2, "prefetch must be 1 or 2");
         require(occupancy == 2 || occupancy == 3, "occupancy must be 2 or 3");
-        b->b->variant = (prefetch - 1) | (occupancy == 2 ? 2 : 0);
+        b->b->variant = spd::batch_variant(rows, prefetch, occupancy);
     });
 }
 
-int spd_batch_get_shape(spd_batch_t b, int* grid, int* interleaved, int* prefetch, int* occupancy)
+int spd_batch_tune_log(spd_batch_t b, spd_tune_entry* out, int max, int* n)
 {
     return guarded([&]
     {
-        require(b && grid && interleaved && prefetch && occupancy, "null argument");
+        require(b && n && (out || max == 0), "null argument");
+        require(max >= 0, "max must be >= 0");
+        const auto& log = b->b->tune_log;
+        const int m = std::min<int>(max, (int)log.size());
+        for (int e = 0; e < m; ++e)
+        {
+            const int v = log[e].variant;
+            out[e] = spd_tune_entry{log[e].grid, log[e].interleaved, spd::batch_variant_rows(v),
+                                    spd::batch_variant_prefetch(v), spd::batch_variant_occupancy(v), log[e].ms_per_step};
+        }
+        *n = m;
+    });
+}
+
+int spd_batch_get_shape(spd_batch_t b, int* grid, int* interleaved, int* rows, int* prefetch, int* occupancy)
+{
+    return guarded([&]
+    {
+        require(b && grid && interleaved && rows && prefetch && occupancy, "null argument");
         *grid = b->b->grid;
         *interleaved = b->b->order_interleaved ? 1 : 0;
+        *rows = spd::batch_variant_rows(b->b->variant);
         *prefetch = spd::batch_variant_prefetch(b->b->variant);
         *occupancy = spd::batch_variant_occupancy(b->b->variant);
     });

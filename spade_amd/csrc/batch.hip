@@ -26,7 +26,7 @@
 namespace spd {
 namespace {
 
-constexpr int HXB = 18, HPB = 18*18;
+constexpr int HXB = 18;
 
 
 struct CellF { float p, T, u, v, w, rho, e; };   // p, T, rho, e are perturbations
@@ -37,41 +37,44 @@ template <int D> __device__ inline float velf(const CellF& c) { return D == 0 ? 
 
 // FP32 face flux of cent_keep<2> + visc_lr about the reference state.
 // Outputs (f32): continuity perturbation, energy perturbation, x/y/z momentum.
-// The FP32 face arithmetic may fuse a*b+c into one FMA: this engine has no
-// bit-for-bit contract with the reference, and a face value is computed once
-// and shared by the two cells it separates, so the FP64 residual sums (kept
-// without contraction) still telescope exactly over a periodic case.
+// The fused multiply-adds are written out and the compiler forms no others,
+// so every inlined copy of a face (interior faces, the 17th x- and y-faces of
+// a tile, the z-faces) rounds the same face identically: a face value is the
+// same for the two cells it separates whichever code evaluated it, the FP64
+// residual sums telescope over a periodic case, and every launch shape gives
+// the same bits.
 template <int D, int LAW>
 __device__ inline void face_f32(const CaseParams& P, const CellF& L, const CellF& R, const float g[3][3], float gT,
                                 float F[5])
 {
-    #pragma clang fp contract(fast)
+    #pragma clang fp contract(off)
     const float unL = velf<D>(L), unR = velf<D>(R);
     const float U   = 0.5f*(unL + unR);
-    const float Fcp = 0.5f*(L.rho + R.rho)*U;                // rho'-part of the mass flux
-    const float Fc  = P.f_rho_ref*U + Fcp;                    // full mass flux
+    const float Fcp = (0.5f*(L.rho + R.rho))*U;              // rho'-part of the mass flux
+    const float Fc  = __builtin_fmaf(P.f_rho_ref, U, Fcp);    // full mass flux
     const float ub[3] = {0.5f*(L.u + R.u), 0.5f*(L.v + R.v), 0.5f*(L.w + R.w)};
-    const float K   = L.u*R.u + L.v*R.v + L.w*R.w;
+    const float K   = __builtin_fmaf(L.w, R.w, __builtin_fmaf(L.v, R.v, L.u*R.u));
     const float eb  = 0.5f*(L.e + R.e);
     float Fm[3] = {Fc*ub[0], Fc*ub[1], Fc*ub[2]};
-    Fm[D] += 0.5f*(L.p + R.p);
-    float FE = 0.5f*Fc*K + Fc*eb + 0.5f*(unL*R.p + unR*L.p);
+    Fm[D] = __builtin_fmaf(0.5f, L.p + R.p, Fm[D]);
+    const float FE = __builtin_fmaf(0.5f*Fc, K, __builtin_fmaf(Fc, eb, 0.5f*__builtin_fmaf(unL, R.p, unR*L.p)));
     // viscous terms
-    const float Tf = P.f_T_ref + 0.5f*(L.T + R.T);
+    const float Tf = __builtin_fmaf(0.5f, L.T + R.T, P.f_T_ref);
     float mu;
     // (T/T_ref)^n as exp2(n log2(T/T_ref)) on the hardware log2 / exp2 (about
     // 1e-7 relative for T near T_ref; OCML's powf costs ~120 instructions)
     if (LAW == SPD_LAW_POWER)
         mu = P.f_mu_ref*__builtin_amdgcn_exp2f(P.f_power*__builtin_amdgcn_logf(Tf*P.f_inv_T_law));
-    else                      mu = P.f_mu_ref;
+    else
+        mu = P.f_mu_ref;
     const float visc2 = -0.6666666666666667f*mu;
     const float cond  = mu*P.f_cp_over_pr;
-    const float div = g[0][0] + g[1][1] + g[2][2];
+    const float div = (g[0][0] + g[1][1]) + g[2][2];
     float t[3];
     #pragma unroll
     for (int b = 0; b < 3; ++b)
-        t[b] = (b == D) ? mu*(2.0f*g[D][D]) + visc2*div : mu*(g[D][b] + g[b][D]);
-    const float ht = ub[0]*t[0] + ub[1]*t[1] + ub[2]*t[2] + cond*gT;
+        t[b] = (b == D) ? __builtin_fmaf(mu, 2.0f*g[D][D], visc2*div) : mu*(g[D][b] + g[b][D]);
+    const float ht = __builtin_fmaf(cond, gT, __builtin_fmaf(ub[2], t[2], __builtin_fmaf(ub[1], t[1], ub[0]*t[0])));
     F[0] = Fcp;
     F[1] = FE - ht;
     F[2] = Fm[0] - t[0];
@@ -81,11 +84,11 @@ __device__ inline void face_f32(const CaseParams& P, const CellF& L, const CellF
 
 __device__ inline float tangf(float a, float b, float c, float d, float it)
 {
-    #pragma clang fp contract(fast)
-    return 0.25f*it*((a - b) + (c - d));
+    return (0.25f*it)*((a - b) + (c - d));
 }
 
 // slot layout of the f32 velocity ring: s[pos] = u, s[HPB + pos] = v, s[2*HPB + pos] = w
+template <int HPB>
 __device__ inline void gradsf_x(const float* sm, const float* s0, const float* sp, int pL, int pR,
                                 const CellF& L, const CellF& R, float ix, float iy, float iz, float g[3][3], float& gT)
 {
@@ -98,6 +101,7 @@ __device__ inline void gradsf_x(const float* sm, const float* s0, const float* s
     g[2][2] = tangf(sp[2*HPB + pL], sm[2*HPB + pL], sp[2*HPB + pR], sm[2*HPB + pR], iz);
 }
 
+template <int HPB>
 __device__ inline void gradsf_y(const float* sm, const float* s0, const float* sp, int pL, int pR,
                                 const CellF& L, const CellF& R, float ix, float iy, float iz, float g[3][3], float& gT)
 {
@@ -110,6 +114,7 @@ __device__ inline void gradsf_y(const float* sm, const float* s0, const float* s
     g[0][2] = 0.0f;
 }
 
+template <int HPB>
 __device__ inline void gradsf_z(const float* sL, const float* sR, int p, const CellF& L, const CellF& R,
                                 float ix, float iy, float iz, float g[3][3], float& gT)
 {
@@ -168,52 +173,74 @@ __device__ inline CaseParams uniform_params(const CaseParams& c)
     return u;
 }
 
-// Launch shapes the autotuner chooses between: PFB planes of conserved state
-// in flight ahead of the one entering the LDS window (1 or 2), and OCC
-// workgroups per CU the register budget is sized for (3: 168 VGPRs with a few
-// spills, 2: 256 VGPRs without; LDS is 52.5 KB either way).
-template <int LAW, bool HAS_U0, int PFB, int OCC>
-__global__ void __launch_bounds__(256, OCC)
+// Launch shapes the autotuner chooses between:
+//   TJ   rows of the block's 16x16 cross-section per workgroup (16: one
+//        256-thread workgroup per block; 8: two 128-thread workgroups, one per
+//        half, each with its own halo rows: more, smaller workgroups per CU);
+//   PFB  planes of conserved state in flight ahead of the one entering the
+//        LDS window (1 or 2);
+//   OCC  waves per SIMD the register budget is sized for (3: 168 VGPRs with a
+//        few spills, 2: 256 VGPRs without).
+// Every shape evaluates the same expressions in the same order, and the
+// per-block diagnostics are summed over the same two half-block trees (rows
+// 0-7 and 8-15, 128 leaves each, added on the host), so all shapes give the
+// same bits.
+template <int TJ> struct Tile
+{
+    static constexpr int NT = 16*TJ;                     // threads
+    static constexpr int HP = HXB*(TJ + 2);              // LDS plane with halo
+    static constexpr int NHALO = 2*TJ + 36;              // halo columns
+    // threads that evaluate the 17th x-faces (i = 16) and y-faces (j = TJ)
+    static constexpr int XB = TJ == 16 ? 128 : 52;
+    static constexpr int YB = TJ == 16 ? 192 : 64;
+};
+
+template <int LAW, bool HAS_U0, int TJ, int PFB, int OCC>
+__global__ void __launch_bounds__(16*TJ, OCC)
 k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q0, double* __restrict__ Qout,
         double alpha, double beta, int want_diag)
 {
-    __shared__ float  sU[3][3*HPB];       // u, v, w (f32) of planes k-1, k, k+1
-    __shared__ float  sS[2][4*HPB];       // p', T', rho', e' of planes k, k+1 (parity)
-    __shared__ double sV[2][3*HPB];       // u, v, w (f64) of planes k, k+1 (parity)
-    __shared__ float  sFx[5][16*17];
-    __shared__ float  sFy[5][17*16];
-    __shared__ double sRed[2][256];
+    using T = Tile<TJ>;
+    constexpr int HP = T::HP, NTILE = 16/TJ;
+    __shared__ float  sU[3][3*HP];        // u, v, w (f32) of planes k-1, k, k+1
+    __shared__ float  sS[2][4*HP];        // p', T', rho', e' of planes k, k+1 (parity)
+    __shared__ double sV[2][3*HP];        // u, v, w (f64) of planes k, k+1 (parity)
+    __shared__ float  sFx[5][TJ*17];
+    __shared__ float  sFy[5][(TJ + 1)*16];
+    __shared__ double sRed[2][T::NT];
 
     const int tid = threadIdx.x;
-    const int i = tid & 15, j = tid >> 4;
+    const int i = tid & 15, j = tid >> 4;                // j: row within the tile
     int hi = 0, hj = 0;
-    const bool has_halo = tid < 68;
-    if (tid < 16)      { hi = -1;       hj = tid; }
-    else if (tid < 32) { hi = 16;       hj = tid - 16; }
-    else if (tid < 48) { hi = tid - 32; hj = -1; }
-    else if (tid < 64) { hi = tid - 48; hj = 16; }
-    else if (tid < 68) { const int c = tid - 64; hi = (c & 1) ? 16 : -1; hj = (c & 2) ? 16 : -1; }
+    const bool has_halo = tid < T::NHALO;
+    if (tid < TJ)             { hi = -1;              hj = tid; }
+    else if (tid < 2*TJ)      { hi = 16;              hj = tid - TJ; }
+    else if (tid < 2*TJ + 16) { hi = tid - 2*TJ;      hj = -1; }
+    else if (tid < 2*TJ + 32) { hi = tid - 2*TJ - 16; hj = TJ; }
+    else if (tid < T::NHALO)  { const int c = tid - 2*TJ - 32; hi = (c & 1) ? 16 : -1; hj = (c & 2) ? TJ : -1; }
     const int pown = lposb(i, j), phalo = lposb(hi, hj);
-    const int nwork = *B.nwork;
+    const int nitems = *B.nwork*NTILE;
 
-    for (int w = blockIdx.x; w < nwork; w += gridDim.x)
+    for (int w = blockIdx.x; w < nitems; w += gridDim.x)
     {
         // slot and case constants are the same for the whole workgroup: held
         // in scalar registers, not reloaded per lane inside the k-march
-        const int slot = __builtin_amdgcn_readfirstlane(B.worklist[w]);
+        const int slot = __builtin_amdgcn_readfirstlane(B.worklist[w/NTILE]);
+        const int j0 = (w % NTILE)*TJ;                   // first row of the tile in the block
         const CaseParams P = uniform_params(B.cases[__builtin_amdgcn_readfirstlane(B.slot_case[slot])]);
         const int* nb = B.nbr + 27*slot;
         const float ix = (float)P.ix, iy = (float)P.iy, iz = (float)P.iz;
+        const int gj = j0 + j, ghj = j0 + hj;            // rows in the block (may be -1 or 16)
         // the slots this thread's two columns (own, halo) read from below the
-        // block, inside it and above it: looked up once per block
+        // block, inside it and above it: looked up once per tile
         auto column_slots = [&](int ci, int cj, int out[3])
         {
             const int exy = (ci < 0 ? 0 : (ci > 15 ? 2 : 1)) + 3*(cj < 0 ? 0 : (cj > 15 ? 2 : 1));
             out[0] = nb[exy]; out[1] = nb[exy + 9]; out[2] = nb[exy + 18];
         };
         int own_slot[3], halo_slot[3];
-        column_slots(i, j, own_slot);
-        column_slots(hi, hj, halo_slot);
+        column_slots(i, gj, own_slot);
+        column_slots(hi, ghj, halo_slot);
 
         auto load_col = [&](const int sl[3], int ci, int cj, int ck) -> Cons
         {
@@ -225,15 +252,15 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
         auto put = [&](int k, int p, const PrimD& q, const CellF& c)
         {
             float* s = sU[slotk(k)];
-            s[p] = c.u; s[HPB + p] = c.v; s[2*HPB + p] = c.w;
+            s[p] = c.u; s[HP + p] = c.v; s[2*HP + p] = c.w;
             float* t = sS[k & 1];
-            t[p] = c.p; t[HPB + p] = c.T; t[2*HPB + p] = c.rho; t[3*HPB + p] = c.e;
+            t[p] = c.p; t[HP + p] = c.T; t[2*HP + p] = c.rho; t[3*HP + p] = c.e;
             double* d = sV[k & 1];
-            d[p] = q.u; d[HPB + p] = q.v; d[2*HPB + p] = q.w;
+            d[p] = q.u; d[HP + p] = q.v; d[2*HP + p] = q.w;
         };
         auto from_lds = [&](const float* t, const float* s0, int p) -> CellF
         {
-            return CellF{t[p], t[HPB + p], s0[p], s0[HPB + p], s0[2*HPB + p], t[2*HPB + p], t[3*HPB + p]};
+            return CellF{t[p], t[HP + p], s0[p], s0[HP + p], s0[2*HP + p], t[2*HP + p], t[3*HP + p]};
         };
 
         // u0 of the plane to be written next (stages 2 and 3), requested before
@@ -241,7 +268,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
         double u0[5];
         auto load_u0 = [&](int k)
         {
-            const long long o = (long long)slot*5*4096 + (k*16 + j)*16 + i;
+            const long long o = (long long)slot*5*4096 + (k*16 + gj)*16 + i;
             #pragma unroll
             for (int n = 0; n < 5; ++n) u0[n] = HAS_U0 ? Q0[o + n*4096] : 0.0;
         };
@@ -250,29 +277,29 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
         // prologue: planes -1, 0, 1 resident, planes 2 .. 1 + PFB in flight.
         // The own column's conserved state of planes k and k+1 stays in
         // registers (the stage input u_in); velocities of planes k, k+1 are in LDS.
-        const PrimD qprev = to_prim(P, load_col(own_slot, i, j, -1));
-        Cons rcur = load_col(own_slot, i, j, 0);
-        Cons rnxt = load_col(own_slot, i, j, 1);
+        const PrimD qprev = to_prim(P, load_col(own_slot, i, gj, -1));
+        Cons rcur = load_col(own_slot, i, gj, 0);
+        Cons rnxt = load_col(own_slot, i, gj, 1);
         const PrimD qcur0 = to_prim(P, rcur), qnxt0 = to_prim(P, rnxt);
         CellF prev = to_f32(P, qprev), cur = to_f32(P, qcur0), nxt = to_f32(P, qnxt0);
         Cons nraw[PFB], hraw[PFB];
         #pragma unroll
-        for (int d = 0; d < PFB; ++d) { nraw[d] = load_col(own_slot, i, j, 2 + d); hraw[d] = Cons{}; }
+        for (int d = 0; d < PFB; ++d) { nraw[d] = load_col(own_slot, i, gj, 2 + d); hraw[d] = Cons{}; }
         {
             float* s = sU[slotk(-1)];
-            s[pown] = prev.u; s[HPB + pown] = prev.v; s[2*HPB + pown] = prev.w;
+            s[pown] = prev.u; s[HP + pown] = prev.v; s[2*HP + pown] = prev.w;
         }
         put(0, pown, qcur0, cur); put(1, pown, qnxt0, nxt);
         if (has_halo)
         {
-            const CellF hp = to_f32(P, to_prim(P, load_col(halo_slot, hi, hj, -1)));
+            const CellF hp = to_f32(P, to_prim(P, load_col(halo_slot, hi, ghj, -1)));
             float* s = sU[slotk(-1)];
-            s[phalo] = hp.u; s[HPB + phalo] = hp.v; s[2*HPB + phalo] = hp.w;
-            const PrimD h0 = to_prim(P, load_col(halo_slot, hi, hj, 0)), h1 = to_prim(P, load_col(halo_slot, hi, hj, 1));
+            s[phalo] = hp.u; s[HP + phalo] = hp.v; s[2*HP + phalo] = hp.w;
+            const PrimD h0 = to_prim(P, load_col(halo_slot, hi, ghj, 0)), h1 = to_prim(P, load_col(halo_slot, hi, ghj, 1));
             put(0, phalo, h0, to_f32(P, h0));
             put(1, phalo, h1, to_f32(P, h1));
             #pragma unroll
-            for (int d = 0; d < PFB; ++d) hraw[d] = load_col(halo_slot, hi, hj, 2 + d);
+            for (int d = 0; d < PFB; ++d) hraw[d] = load_col(halo_slot, hi, ghj, 2 + d);
         }
         double ke_acc = 0.0, en_acc = 0.0;
         double wprev_u = qprev.u, wprev_v = qprev.v, wprev_w = qprev.w;   // own column, plane k-1 (f64)
@@ -281,13 +308,13 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
         float zlo[5];
         {
             float g[3][3], gT;
-            gradsf_z(sU[slotk(-1)], sU[slotk(0)], pown, prev, cur, ix, iy, iz, g, gT);
+            gradsf_z<HP>(sU[slotk(-1)], sU[slotk(0)], pown, prev, cur, ix, iy, iz, g, gT);
             face_f32<2, LAW>(P, prev, cur, g, gT, zlo);
         }
 
         for (int k = 0; k < 16; ++k)
         {
-            const long long o = (long long)slot*5*4096 + (k*16 + j)*16 + i;
+            const long long o = (long long)slot*5*4096 + (k*16 + gj)*16 + i;
             const float* sm = sU[slotk(k - 1)];
             const float* s0 = sU[slotk(k)];
             const float* sp = sU[slotk(k + 1)];
@@ -296,7 +323,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 const int pL = pown - 1;
                 const CellF L = from_lds(sk, s0, pL);
                 float g[3][3], gT, F[5];
-                gradsf_x(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
+                gradsf_x<HP>(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
                 face_f32<0, LAW>(P, L, cur, g, gT, F);
                 for (int n = 0; n < 5; ++n) sFx[n][j*17 + i] = F[n];
             }
@@ -304,54 +331,54 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 const int pL = pown - HXB;
                 const CellF L = from_lds(sk, s0, pL);
                 float g[3][3], gT, F[5];
-                gradsf_y(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
+                gradsf_y<HP>(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
                 face_f32<1, LAW>(P, L, cur, g, gT, F);
                 for (int n = 0; n < 5; ++n) sFy[n][j*16 + i] = F[n];
             }
-            if (tid >= 128 && tid < 144)
+            if (tid >= T::XB && tid < T::XB + TJ)
             {
-                const int jj = tid - 128;
+                const int jj = tid - T::XB;
                 const int pL = lposb(15, jj), pR = lposb(16, jj);
                 const CellF L = from_lds(sk, s0, pL), Rr = from_lds(sk, s0, pR);
                 float g[3][3], gT, F[5];
-                gradsf_x(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
+                gradsf_x<HP>(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
                 face_f32<0, LAW>(P, L, Rr, g, gT, F);
                 for (int n = 0; n < 5; ++n) sFx[n][jj*17 + 16] = F[n];
             }
-            if (tid >= 192 && tid < 208)
+            if (tid >= T::YB && tid < T::YB + 16)
             {
-                const int ii = tid - 192;
-                const int pL = lposb(ii, 15), pR = lposb(ii, 16);
+                const int ii = tid - T::YB;
+                const int pL = lposb(ii, TJ - 1), pR = lposb(ii, TJ);
                 const CellF L = from_lds(sk, s0, pL), Rr = from_lds(sk, s0, pR);
                 float g[3][3], gT, F[5];
-                gradsf_y(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
+                gradsf_y<HP>(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
                 face_f32<1, LAW>(P, L, Rr, g, gT, F);
-                for (int n = 0; n < 5; ++n) sFy[n][16*16 + ii] = F[n];
+                for (int n = 0; n < 5; ++n) sFy[n][TJ*16 + ii] = F[n];
             }
             float zhi[5];
             {
                 float g[3][3], gT;
-                gradsf_z(s0, sp, pown, cur, nxt, ix, iy, iz, g, gT);
+                gradsf_z<HP>(s0, sp, pown, cur, nxt, ix, iy, iz, g, gT);
                 face_f32<2, LAW>(P, cur, nxt, g, gT, zhi);
             }
             // FP64 velocity differences of the cell (planes k-1..k+1)
             const double* vk = sV[k & 1];
             const double dux = vk[pown + 1] - vk[pown - 1];                       // u(i+1) - u(i-1)
-            const double dvy = vk[HPB + pown + HXB] - vk[HPB + pown - HXB];        // v(j+1) - v(j-1)
+            const double dvy = vk[HP + pown + HXB] - vk[HP + pown - HXB];          // v(j+1) - v(j-1)
             const double* vn = sV[(k + 1) & 1];
-            const double dwz = vn[2*HPB + pown] - wprev_w;                         // w(k+1) - w(k-1)
+            const double dwz = vn[2*HP + pown] - wprev_w;                          // w(k+1) - w(k-1)
             const double DU = -0.5*(dux*P.ix + dvy*P.iy + dwz*P.iz);               // -div u
             if (want_diag)
             {
-                const double dwy = vk[2*HPB + pown + HXB] - vk[2*HPB + pown - HXB];
+                const double dwy = vk[2*HP + pown + HXB] - vk[2*HP + pown - HXB];
                 const double duy = vk[pown + HXB] - vk[pown - HXB];
-                const double dvx = vk[HPB + pown + 1] - vk[HPB + pown - 1];
-                const double dwx = vk[2*HPB + pown + 1] - vk[2*HPB + pown - 1];
-                const double dvz = vn[HPB + pown] - wprev_v, duz = vn[pown] - wprev_u;
+                const double dvx = vk[HP + pown + 1] - vk[HP + pown - 1];
+                const double dwx = vk[2*HP + pown + 1] - vk[2*HP + pown - 1];
+                const double dvz = vn[HP + pown] - wprev_v, duz = vn[pown] - wprev_u;
                 const double ox = 0.5*(dwy*P.iy - dvz*P.iz);
                 const double oy = 0.5*(duz*P.iz - dwx*P.ix);
                 const double oz = 0.5*(dvx*P.ix - duy*P.iy);
-                const double uc = vk[pown], vc = vk[HPB + pown], wc = vk[2*HPB + pown];
+                const double uc = vk[pown], vc = vk[HP + pown], wc = vk[2*HP + pown];
                 ke_acc += 0.5*rcur.r*(uc*uc + vc*vc + wc*wc);
                 en_acc += 0.5*(ox*ox + oy*oy + oz*oz);
             }
@@ -381,7 +408,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             }
 
             // advance the window
-            wprev_u = vk[pown]; wprev_v = vk[HPB + pown]; wprev_w = vk[2*HPB + pown];
+            wprev_u = vk[pown]; wprev_v = vk[HP + pown]; wprev_w = vk[2*HP + pown];
             if (k + 1 < 16)
             {
                 load_u0(k + 1);
@@ -391,32 +418,34 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 put(k + 2, pown, q2, nxt);
                 #pragma unroll
                 for (int d = 0; d + 1 < PFB; ++d) nraw[d] = nraw[d + 1];
-                if (k + 2 + PFB <= 16) nraw[PFB - 1] = load_col(own_slot, i, j, k + 2 + PFB);
+                if (k + 2 + PFB <= 16) nraw[PFB - 1] = load_col(own_slot, i, gj, k + 2 + PFB);
                 if (has_halo)
                 {
                     const PrimD hq = to_prim(P, hraw[0]);
                     put(k + 2, phalo, hq, to_f32(P, hq));
                     #pragma unroll
                     for (int d = 0; d + 1 < PFB; ++d) hraw[d] = hraw[d + 1];
-                    if (k + 2 + PFB <= 16) hraw[PFB - 1] = load_col(halo_slot, hi, hj, k + 2 + PFB);
+                    if (k + 2 + PFB <= 16) hraw[PFB - 1] = load_col(halo_slot, hi, ghj, k + 2 + PFB);
                 }
             }
             lds_barrier();
         }
         if (want_diag)
         {
+            // one 128-leaf tree per half block (rows 0-7, 8-15), the same in every shape
             sRed[0][tid] = ke_acc; sRed[1][tid] = en_acc;
             lds_barrier();
-            for (int s = 128; s > 0; s >>= 1)
+            for (int s = 64; s > 0; s >>= 1)
             {
-                if (tid < s) { sRed[0][tid] += sRed[0][tid + s]; sRed[1][tid] += sRed[1][tid + s]; }
+                if ((tid & 127) < s) { sRed[0][tid] += sRed[0][tid + s]; sRed[1][tid] += sRed[1][tid + s]; }
                 lds_barrier();
             }
-            if (tid == 0)
+            if ((tid & 127) == 0)
             {
+                const int half = (j0 + j) >> 3;
                 const int h = *B.step % B.hist_len;
-                B.diag[((long long)h*B.max_slots + slot)*2 + 0] = sRed[0][0];
-                B.diag[((long long)h*B.max_slots + slot)*2 + 1] = sRed[1][0];
+                B.diag[(((long long)h*B.max_slots + slot)*2 + half)*2 + 0] = sRed[0][tid];
+                B.diag[(((long long)h*B.max_slots + slot)*2 + half)*2 + 1] = sRed[1][tid];
             }
             lds_barrier();
         }
@@ -480,32 +509,54 @@ k_init_case(const int* __restrict__ stage, int nslots, CaseParams P, CaseIC ic, 
 
 namespace {
 
-// shape variant v: PFB = 1 + (v & 1), OCC = (v & 2) ? 2 : 3
+// shape variant v: PFB = 1 + (v & 1), OCC = (v & 2) ? 2 : 3, TJ = (v & 4) ? 8 : 16
+template <int LAW, bool HAS_U0, int TJ, int PFB, int OCC>
+void launch_one(int grid, hipStream_t s, const BatchDev& B, const double* Qin, const double* Q0, double* Qout,
+                double alpha, double beta, int want_diag)
+{
+    hipLaunchKernelGGL((k_stage<LAW, HAS_U0, TJ, PFB, OCC>), dim3(grid), dim3(16*TJ), 0, s,
+                       B, Qin, Q0, Qout, alpha, beta, want_diag);
+}
+
 template <int LAW, bool HAS_U0>
 void launch_shape(int v, const BatchDev& B, const double* Qin, const double* Q0, double* Qout,
                   double alpha, double beta, int want_diag, int grid, hipStream_t s)
 {
     switch (v)
     {
-    case 0:  hipLaunchKernelGGL((k_stage<LAW, HAS_U0, 1, 3>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
-    case 1:  hipLaunchKernelGGL((k_stage<LAW, HAS_U0, 2, 3>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
-    case 2:  hipLaunchKernelGGL((k_stage<LAW, HAS_U0, 1, 2>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
-    default: hipLaunchKernelGGL((k_stage<LAW, HAS_U0, 2, 2>), dim3(grid), dim3(256), 0, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
+    case 0:  launch_one<LAW, HAS_U0, 16, 1, 3>(grid, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
+    case 1:  launch_one<LAW, HAS_U0, 16, 2, 3>(grid, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
+    case 2:  launch_one<LAW, HAS_U0, 16, 1, 2>(grid, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
+    case 3:  launch_one<LAW, HAS_U0, 16, 2, 2>(grid, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
+    case 4:  launch_one<LAW, HAS_U0, 8, 1, 3>(grid, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
+    case 5:  launch_one<LAW, HAS_U0, 8, 2, 3>(grid, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
+    case 6:  launch_one<LAW, HAS_U0, 8, 1, 2>(grid, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
+    default: launch_one<LAW, HAS_U0, 8, 2, 2>(grid, s, B, Qin, Q0, Qout, alpha, beta, want_diag); break;
     }
+}
+
+template <int LAW, int TJ, int PFB, int OCC>
+int occ_one()
+{
+    int n = 0;
+    SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<LAW, true, TJ, PFB, OCC>, 16*TJ, 0));
+    return n;
 }
 
 template <int LAW>
 int occupancy(int v)
 {
-    int n = 0;
     switch (v)
     {
-    case 0:  SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<LAW, true, 1, 3>, 256, 0)); break;
-    case 1:  SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<LAW, true, 2, 3>, 256, 0)); break;
-    case 2:  SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<LAW, true, 1, 2>, 256, 0)); break;
-    default: SPD_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<LAW, true, 2, 2>, 256, 0)); break;
+    case 0:  return occ_one<LAW, 16, 1, 3>();
+    case 1:  return occ_one<LAW, 16, 2, 3>();
+    case 2:  return occ_one<LAW, 16, 1, 2>();
+    case 3:  return occ_one<LAW, 16, 2, 2>();
+    case 4:  return occ_one<LAW, 8, 1, 3>();
+    case 5:  return occ_one<LAW, 8, 2, 3>();
+    case 6:  return occ_one<LAW, 8, 1, 2>();
+    default: return occ_one<LAW, 8, 2, 2>();
     }
-    return n;
 }
 
 } // namespace

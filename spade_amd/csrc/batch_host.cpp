@@ -24,7 +24,7 @@ Batch::Batch(Ctx* c, long long budget_bytes, int law_in, int hist) : ctx(c), law
     need(law == SPD_LAW_CONST || law == SPD_LAW_POWER, "unknown viscosity law");
     need(hist_len >= 1 && hist_len <= 4096, "history length must be in 1..4096");
     const size_t per_slot = 3*kSlotDoubles*sizeof(double) + 27*sizeof(int) + 2*sizeof(int)
-                          + (size_t)hist_len*2*sizeof(double);
+                          + (size_t)hist_len*4*sizeof(double);
     need(budget_bytes > (long long)per_slot, "HBM budget smaller than one block");
     max_slots = (int)std::min<long long>(budget_bytes/(long long)per_slot, 1 << 22);
     max_cases = std::max(1, max_slots);
@@ -37,10 +37,10 @@ Batch::Batch(Ctx* c, long long budget_bytes, int law_in, int hist) : ctx(c), law
     SPD_HIP(hipMalloc((void**)&d_nbr, sizeof(int)*27*(size_t)max_slots));
     SPD_HIP(hipMalloc((void**)&d_small, sizeof(int)*64));
     SPD_HIP(hipMalloc((void**)&d_cases, sizeof(CaseParams)*(size_t)max_cases));
-    SPD_HIP(hipMalloc((void**)&d_diag, sizeof(double)*2*(size_t)hist_len*max_slots));
+    SPD_HIP(hipMalloc((void**)&d_diag, sizeof(double)*4*(size_t)hist_len*max_slots));
     SPD_HIP(hipMalloc((void**)&d_init_slots, sizeof(int)*29*(size_t)max_slots));
     SPD_HIP(hipMemsetAsync(d_small, 0, sizeof(int)*64, ctx->stream));
-    SPD_HIP(hipMemsetAsync(d_diag, 0, sizeof(double)*2*(size_t)hist_len*max_slots, ctx->stream));
+    SPD_HIP(hipMemsetAsync(d_diag, 0, sizeof(double)*4*(size_t)hist_len*max_slots, ctx->stream));
     h_slot_case.assign(max_slots, -1);
     h_nbr.assign(27*(size_t)max_slots, 0);
     h_cases.resize(max_cases);
@@ -248,14 +248,15 @@ void Batch::collect_diag(int first_step, int nsteps)
     size_t hi = 0;
     for (auto& kv : cases)
         for (int s : kv.second.slots) hi = std::max(hi, (size_t)s + 1);
-    std::vector<double> rows((size_t)nsteps*2*hi);
+    // per slot: {ke, enstrophy} of rows 0-7, then of rows 8-15
+    std::vector<double> rows((size_t)nsteps*4*hi);
     if (hi > 0)
     {
         for (int t = 0; t < nsteps; ++t)
         {
             const int h = (first_step + t) % hist_len;
-            SPD_HIP(hipMemcpyAsync(rows.data() + (size_t)t*2*hi, d_diag + (size_t)h*2*max_slots,
-                                   sizeof(double)*2*hi, hipMemcpyDeviceToHost, ctx->stream));
+            SPD_HIP(hipMemcpyAsync(rows.data() + (size_t)t*4*hi, d_diag + (size_t)h*4*max_slots,
+                                   sizeof(double)*4*hi, hipMemcpyDeviceToHost, ctx->stream));
         }
     }
     SPD_HIP(hipStreamSynchronize(ctx->stream));
@@ -268,8 +269,9 @@ void Batch::collect_diag(int first_step, int nsteps)
             double ke = 0.0, en = 0.0;
             for (int s : c.slots)   // fixed (case block) order: identical alone or in a mix
             {
-                ke += rows[(size_t)t*2*hi + 2*(size_t)s + 0];
-                en += rows[(size_t)t*2*hi + 2*(size_t)s + 1];
+                const double* r = rows.data() + (size_t)t*4*hi + 4*(size_t)s;
+                ke += r[0] + r[2];
+                en += r[1] + r[3];
             }
             c.ke.push_back(ke/ncell);
             c.en.push_back(en/ncell);
@@ -340,8 +342,9 @@ void Batch::upload(int id, const double* host)
 
 // Autotuner: times one step for each candidate launch shape on the current
 // batch and keeps the fastest.  Candidates: the stage kernel's shape variant
-// (1 or 2 planes of state in flight; registers sized for 3 or 2 workgroups per
-// CU), the persistent grid (1..4 workgroups per CU) and the work order
+// (16 or 8 rows of a block's cross-section per workgroup; 1 or 2 planes of
+// state in flight; registers sized for 3 or 2 waves per SIMD), the persistent
+// grid (1..4 blocks' worth of workgroups per CU) and the work order
 // (case-major or interleaved).  State is saved and restored around the trials
 // so tuning does not advance any case.
 int Batch::tune()
@@ -368,9 +371,11 @@ int Batch::tune()
         for (int v = 0; v < kBatchShapeVariants; ++v)
         {
             variant = v;
+            // 1..4 block-sized workgroups per CU (two half-block workgroups count as one)
+            const int per_block = 16/batch_variant_rows(v);
             for (int per_cu = 1; per_cu <= 4; ++per_cu)
             {
-                grid = per_cu*num_cu;
+                grid = per_cu*per_block*num_cu;
                 enqueue_step(ctx->stream);              // warm-up
                 SPD_HIP(hipEventRecord(ctx->ev0, ctx->stream));
                 for (int r = 0; r < 2; ++r) enqueue_step(ctx->stream);

@@ -104,8 +104,10 @@ def run_ensemble(specs, steps, tune=True):
     blocks = sum(s.blocks for s in specs)
     b = Batch(int(blocks * BYTES_PER_BLOCK * 1.02) + (64 << 20))
     ids = [b.add(s) for s in specs]
+    tlog = []
     if tune:
         b.tune()
+        tlog = b.tune_log()
     shape = b.shape()
     b.step(1)
     b.ctx.sync()
@@ -119,7 +121,8 @@ def run_ensemble(specs, steps, tune=True):
     alg_bytes = cells * steps * (80.0 + 120.0 + 120.0)
     out = {"cases": len(specs), "blocks": blocks, "cells": cells, "steps": steps, "seconds": dt,
            "cell_updates_per_s": cells * steps / dt, "mcell_rhs_per_s": 3.0 * cells * steps / dt * 1e-6,
-           "hbm_algorithmic_gb_s": alg_bytes / dt * 1e-9, "hbm_frac": alg_bytes / dt / 8e12, "tuned_shape": shape}
+           "hbm_algorithmic_gb_s": alg_bytes / dt * 1e-9, "hbm_frac": alg_bytes / dt / 8e12, "tuned_shape": shape,
+           "tune_log": tlog}
     b.close()
     del ids
     return out

@@ -114,20 +114,24 @@ def test_alone_equals_mix_bitwise():
     assert np.array_equal(ke, ref_ke) and np.array_equal(en, ref_en)
 
 
-@pytest.mark.parametrize("prefetch,occupancy", [(1, 3), (2, 3), (1, 2), (2, 2)])
-def test_kernel_shapes_agree_bitwise(prefetch, occupancy):
+SHAPES = [(r, p, o) for r in (16, 8) for p in (1, 2) for o in (3, 2)]
+
+
+@pytest.mark.parametrize("rows,prefetch,occupancy", SHAPES)
+def test_kernel_shapes_agree_bitwise(rows, prefetch, occupancy):
     spec = sa.CaseSpec("vortex", (32, 16, 32), mach=0.3, reynolds=700.0)
-    ref_state, (ref_ke, _) = _run_alone(spec, 6)
+    ref_state, (ref_ke, ref_en) = _run_alone(spec, 6)
     b = sa.Batch(256 * MB)
-    b.set_kernel_shape(prefetch, occupancy)
-    b.set_shape(768, interleaved=True)
+    b.set_kernel_shape(rows, prefetch, occupancy)
+    b.set_shape(3 * 256 * (16 // rows), interleaved=True)
     cid = b.add(spec)
     b.add(_tgv(16, 4))
     b.step(6)
     sh = b.shape()
-    assert (sh["prefetch"], sh["occupancy"]) == (prefetch, occupancy), sh
+    assert (sh["rows"], sh["prefetch"], sh["occupancy"]) == (rows, prefetch, occupancy), sh
     assert np.array_equal(b.download(cid), ref_state)
-    assert np.array_equal(b.diag(cid)[0], ref_ke)
+    ke, en = b.diag(cid)
+    assert np.array_equal(ke, ref_ke) and np.array_equal(en, ref_en)
 
 
 def test_tuner_reports_a_candidate_shape():
@@ -137,9 +141,14 @@ def test_tuner_reports_a_candidate_shape():
     g, inter = b.tune()
     sh = b.shape()
     assert sh["grid"] == g and sh["interleaved"] == inter
-    assert sh["prefetch"] in (1, 2) and sh["occupancy"] in (2, 3)
-    with pytest.raises(sa.SpadeError):
-        b.set_kernel_shape(3, 3)
+    log = b.tune_log()
+    assert len(log) == 2 * 8 * 4                       # orders x kernel shapes x grids
+    best = min(log, key=lambda e: e["ms_per_step"])
+    assert {k: best[k] for k in sh} == sh
+    assert sh["rows"] in (16, 8) and sh["prefetch"] in (1, 2) and sh["occupancy"] in (2, 3)
+    for bad in ((16, 3, 3), (12, 1, 3), (8, 1, 4)):
+        with pytest.raises(sa.SpadeError):
+            b.set_kernel_shape(*bad)
 
 
 def test_graph_and_direct_launch_agree_bitwise():
