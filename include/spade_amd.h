@@ -202,6 +202,15 @@ int spd_integrator_advance(spd_integrator_t it, int nsteps);
 int spd_integrator_destroy(spd_integrator_t it);
 /* Residual array k_i of the integrator (0-based), for inspection. */
 spd_array_t spd_integrator_residual(spd_integrator_t it, int i);
+/* *fused = 1 when spd_integrator_advance runs each SSP-RK3 stage as one
+ * launch of flux_div fused with the stage update (the fast kernel's
+ * compositions on 16 x 16 x nz blocks, nz >= 2 ng, ng <= 2, every axis
+ * periodic or walled on both faces; the step then ping-pongs between the
+ * state and a second buffer the integrator owns, and the array handle holds
+ * the new state after each step), 0 when it runs separate flux_div and stage
+ * launches.  The results are the same bits either way.  No counterpart in
+ * the reference (integrator.h:18-59 has one path). */
+int spd_integrator_fused(spd_integrator_t it, int* fused);
 
 /* ---- explicit Runge-Kutta with any Butcher table and caller-supplied
  *      residual / boundary callbacks: integrator_t(axis, scheme, data, rhs, bc,

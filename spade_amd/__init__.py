@@ -477,6 +477,14 @@ class Integrator:
     def advance(self, nsteps=1):
         check(load().spd_integrator_advance(self._h, int(nsteps)))
 
+    @property
+    def fused(self):
+        """True when each SSP-RK3 stage runs as one fused flux_div + stage launch
+        (same bits as the separate launches)."""
+        f = C.c_int()
+        check(load().spd_integrator_fused(self._h, C.byref(f)))
+        return bool(f.value)
+
     def solution(self):
         return self.q
 

@@ -121,6 +121,7 @@ SIGNATURES = {
     "spd_boundary_fill": (C.c_int, [_vp, C.c_int, C.POINTER(FillKernel)]),
     "spd_integrator_create": (C.c_int, [_vp, C.POINTER(FluxDesc), C.POINTER(BCDesc), C.c_int, C.c_double,
                                         C.POINTER(_vp)]),
+    "spd_integrator_fused": (C.c_int, [_vp, C.POINTER(C.c_int)]),
     "spd_integrator_advance": (C.c_int, [_vp, C.c_int]),
     "spd_integrator_destroy": (C.c_int, [_vp]),
     "spd_integrator_residual": (_vp, [_vp, C.c_int]),

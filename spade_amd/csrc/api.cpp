@@ -5,6 +5,7 @@
 #include <cstring>
 #include <cstdlib>
 #include <memory>
+#include <utility>
 
 namespace spd {
 
@@ -96,11 +97,29 @@ double* ensure_stage(Grid& g)
 // The boundary step after a stage (exchange + wall fill): the stage kernels
 // have already written the exchanged ghosts (push_ghosts), so only the walls
 // remain; they mirror cells that include exchanged x-ghosts, hence after.
-void apply_bc(spd_integrator_s* it)
+void apply_bc(spd_integrator_s* it, double* a)
 {
     Grid& g = *it->q->a.grid;
     for (int face = 0; face < 6; ++face)
-        if (it->bc.wall_mask & (1 << face)) launch_wall_fill(g, it->q->a.d, face, it->bc.t_wall, g.ctx->stream);
+        if (it->bc.wall_mask & (1 << face)) launch_wall_fill(g, a ? a : it->q->a.d, face, it->bc.t_wall, g.ctx->stream);
+}
+
+// The SSP-RK3 step runs as three fused flux + stage launches (ping-pong
+// between the state and a second buffer) when the fast kernel takes the
+// composition and every ghost cell of a stage's output is rewritten by that
+// stage: each axis periodic (the fused exchange) or walled on both faces
+// (the wall fill covers its whole ghost slab).  SPD_DISABLE_FUSED=1 keeps
+// the separate flux_div and stage launches.
+static bool fused_step_ok(spd_integrator_s* it)
+{
+    if (const char* e = getenv("SPD_DISABLE_FUSED")) if (e[0] == '1') return false;
+    Grid& g = *it->q->a.grid;
+    for (int a = 0; a < 3; ++a)
+    {
+        const bool walls = (it->bc.wall_mask & (1 << (2*a))) && (it->bc.wall_mask & (1 << (2*a + 1)));
+        if (!g.desc.periodic[a] && !walls) return false;
+    }
+    return flux_stage_fast_eligible(g, it->flux);
 }
 
 spd_array_s* new_array(Grid& g)
@@ -491,7 +510,39 @@ int spd_integrator_advance(spd_integrator_t it, int nsteps)
         const double gamma = it->flux.gamma, R = it->flux.R;
         for (int n = 0; n < nsteps; ++n)
         {
-            if (it->scheme == SPD_SCHEME_SSPRK3)
+            if (it->scheme == SPD_SCHEME_SSPRK3 && fused_step_ok(it))
+            {
+                // the stages of the branch below, each fused with its flux_div:
+                // q -> alt -> q -> alt, then the two buffers trade places so the
+                // array handle holds the new state (the handle is opaque).  On
+                // periodic identity grids a stage reads its halo from the
+                // neighbour blocks' interiors, so only the last stage of this
+                // call writes ghost cells (the first reads the caller's, as
+                // the separate launches would)
+                if (!it->alt) it->alt = new_array(g);
+                const bool nb = stage_reads_neighbours_ok(g);
+                double* a = q;
+                double* b = it->alt->a.d;
+                double* r[3] = {it->r[0]->a.d, it->r[1]->a.d, it->r[2]->a.d};
+                const double* rp[2] = {r[0], r[1]};
+                const bool first = n == 0, last = n + 1 == nsteps;
+                { const double c[1] = {(1.0/1.0)*dt};
+                  launch_flux_stage_fast(g, a, b, r[0], it->flux, rp, c, 1, true, gamma, R, g.ctx->stream,
+                                         nb && !first, !nb); }
+                apply_bc(it, b);
+                { const double c[2] = {(-3.0/4.0)*dt, (1.0/4.0)*dt};
+                  launch_flux_stage_fast(g, b, a, r[1], it->flux, rp, c, 2, true, gamma, R, g.ctx->stream,
+                                         nb, !nb); }
+                apply_bc(it, a);
+                { const double c[3] = {(-1.0/12.0)*dt, (-1.0/12.0)*dt, (2.0/3.0)*dt};
+                  launch_flux_stage_fast(g, a, b, r[2], it->flux, rp, c, 3, false, gamma, R, g.ctx->stream,
+                                         nb, !nb || last); }
+                apply_bc(it, b);
+                std::swap(it->q->a.base, it->alt->a.base);
+                std::swap(it->q->a.d, it->alt->a.d);
+                q = it->q->a.d;
+            }
+            else if (it->scheme == SPD_SCHEME_SSPRK3)
             {
                 // rows a = {(0,0,0), (1,0,0), (1/4,1/4,0)}, b = (1/6,1/6,2/3)
                 // (time-integration/explicit.h:80-88); stage coefficients are
@@ -528,6 +579,15 @@ int spd_integrator_advance(spd_integrator_t it, int nsteps)
     });
 }
 
+int spd_integrator_fused(spd_integrator_t it, int* fused)
+{
+    return guarded([&]
+    {
+        require(it && fused, "null argument");
+        *fused = (!it->generic && it->scheme == SPD_SCHEME_SSPRK3 && fused_step_ok(it)) ? 1 : 0;
+    });
+}
+
 int spd_integrator_destroy(spd_integrator_t it)
 {
     return guarded([&]
@@ -536,6 +596,7 @@ int spd_integrator_destroy(spd_integrator_t it)
         for (auto* r : it->r) free_array(r);
         for (auto* r : it->res) free_array(r);
         free_array(it->q2);
+        free_array(it->alt);
         delete it;
     });
 }

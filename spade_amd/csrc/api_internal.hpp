@@ -27,6 +27,7 @@ struct spd_integrator_s
     bool has_flux = false, has_walls = false;
     spd_array_s* res[SPD_RK_MAX_STAGES] = {};
     spd_array_s* q2 = nullptr;          // second solution copy (high storage, identity transform)
+    spd_array_s* alt = nullptr;         // fused SSP-RK3 stages: the other state buffer (ping-pong)
 };
 
 namespace spd {
@@ -71,7 +72,7 @@ void free_array(spd_array_s* A);
 // flux_div with the kernel the library selects
 void run_flux_div(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f, bool increment);
 // The wall fills of the fixed integrators after a stage (the exchange is fused into the stage)
-void apply_bc(spd_integrator_s* it);
+void apply_bc(spd_integrator_s* it, double* a = nullptr);   // a: the array to fill (default: the state)
 // One step of a general-table integrator
 void advance_generic(spd_integrator_s* it);
 

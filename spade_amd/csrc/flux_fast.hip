@@ -28,6 +28,7 @@
 // rounding.
 #include "spd_internal.hpp"
 #include "pow_series.hpp"
+#include "spd_stage.hpp"
 
 namespace spd {
 namespace {
@@ -355,12 +356,39 @@ __device__ inline int lpos(int i, int j) { return (j + 1)*HX + (i + 1); }
 
 struct Raw { double p, T, u, v, w; };
 
-template <int KEEP, int PDISS, int VISC, bool INC, bool MAP>
+// The RK stage a fused launch applies to each cell once its residual L is
+// known (transform-aware, time-integration/advance.h:56-102):
+//   qout = c2p(p2c(q) + c[0] r[0] + .. + c[n-2] r[n-2] + c[n-1] L)
+// in k_rk_transform's order (nonzero coefficients only), then pushed into the
+// ghosts of qout it feeds.
+struct StageArgs
+{
+    double* qout;
+    const double* r[2];          // residuals of the earlier stages, rhs layout
+    double c[3];
+    int n;                       // 1..3 coefficients, the last one for L
+    int store_rhs;               // also write L to rhs
+    double gamma, R;
+    const int* push;             // push table (d_push)
+    int push_ghosts;             // write the ghosts this stage's result feeds
+    const int* nbr;              // source table (d_nbr), read halos from neighbour interiors (STG == 2)
+};
+
+// STG: 0 flux_div only; 1 fused stage, halo from this block's ghost cells;
+// 2 fused stage, halo from the neighbour blocks' interiors (identity
+// coordinates; every ghost region has a source block)
+template <int KEEP, int PDISS, int VISC, bool INC, bool MAP, int STG = 0>
 __global__ void __launch_bounds__(256, 2)
 k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, FastCfg C,
-            int* __restrict__ start)
+            int* __restrict__ start, StageArgs A)
 {
     using S = Spec<KEEP, PDISS, VISC>;
+    static_assert(!(STG && INC), "a fused stage writes a fresh residual");
+    static_assert(!(STG == 2 && MAP), "neighbour-read stages are built for identity coordinates");
+    // x-ghost pushes of a fused stage: the plane's cells within ng of an x
+    // face, staged for whole-row writes (flushed after the next barrier)
+    __shared__ double sXs[STG ? sizeof(XStage)/sizeof(double) : 1];
+    XStage& xs = *reinterpret_cast<XStage*>(sXs);
     __shared__ int sBad[2];               // a face of plane k is outside the pow series range (by parity)
     // mapped coordinates: d x_a / d xi_a of this block at cell centres -1 .. 16
     // and at faces 0 .. 16 of each axis, from the same map_deriv as the
@@ -412,13 +440,35 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     const double* __restrict__ Bown = prims + blk + stored_offset(G, i, j, 0);
     const double* __restrict__ Bhalo = prims + blk + stored_offset(G, hi, hj, 0);
     const long long hplane = (hi >= 0 && hi < 16) ? plane : xplane;
-    double* __restrict__ Rown = rhs + blk + stored_offset(G, i, j, 0);
-
-    auto load_at = [&](const double* __restrict__ col, int ck, long long stride) -> Raw
+    // STG == 2: the halo is read from the neighbour blocks' interiors (d_nbr:
+    // the block that feeds each ghost region), never from this block's ghost
+    // cells; the halo column is then a core column of its x/y neighbour, and
+    // planes -1 and nz come from the z neighbours (the same values an
+    // exchange would have copied into the ghosts)
+    const int hex = hi < 0 ? -1 : (hi > 15 ? 1 : 0), hey = hj < 0 ? -1 : (hj > 15 ? 1 : 0);
+    const int* __restrict__ nbl = STG == 2 ? A.nbr + 27*lb : nullptr;
+    const double* __restrict__ Bhalo_n = (STG == 2 && has_halo)
+        ? prims + (long long)nbl[(hex + 1) + 3*(hey + 1) + 9]*G.sb + stored_offset(G, hi - 16*hex, hj - 16*hey, 0)
+        : Bhalo;
+    auto load_ptr = [&](const double* e) -> Raw { return Raw{e[0], e[sv], e[2*sv], e[3*sv], e[4*sv]}; };
+    auto load_own = [&](int ck) -> Raw
     {
-        const double* e = col + ck*stride;
-        return Raw{e[0], e[sv], e[2*sv], e[3*sv], e[4*sv]};
+        if (STG != 2 || (ck >= 0 && ck < nz)) return load_ptr(Bown + ck*plane);
+        const int ez = ck < 0 ? -1 : 1;
+        const long long S = nbl[1 + 3 + 9*(ez + 1)];
+        return load_ptr(prims + S*G.sb + stored_offset(G, i, j, ck - ez*nz));
     };
+    auto load_halo = [&](int ck) -> Raw
+    {
+        if (STG != 2) return load_ptr(Bhalo + ck*hplane);
+        if (ck >= 0 && ck < nz) return load_ptr(Bhalo_n + ck*plane);
+        const int ez = ck < 0 ? -1 : 1;
+        const long long S = nbl[(hex + 1) + 3*(hey + 1) + 9*(ez + 1)];
+        return load_ptr(prims + S*G.sb + stored_offset(G, hi - 16*hex, hj - 16*hey, ck - ez*nz));
+    };
+    const long long own_off = blk + stored_offset(G, i, j, 0);   // core column (i, j) at k = 0
+    double* __restrict__ Rown = rhs + own_off;
+
     auto cell = [&](const Raw& r) -> CellV { return make_cell(C, r.p, r.T, r.u, r.v, r.w); };
     // LDS window: ring slots of velocities (rotated by pointer), scalar buffers by parity
     double* ring[3] = {sU[0], sU[1], sU[2]};      // planes k-1, k, k+1
@@ -438,24 +488,24 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     // prologue: planes -1, 0, 1 resident; planes 2 and 3 in flight (two raw
     // register sets used alternately, so a plane's load is never copied and
     // waiting for it never waits for the later one)
-    const CellV prev = cell(load_at(Bown, -1, plane));
-    const CellV cur0 = cell(load_at(Bown, 0, plane));
-    const CellV nxt0 = cell(load_at(Bown, 1, plane));
+    const CellV prev = cell(load_own(-1));
+    const CellV cur0 = cell(load_own(0));
+    const CellV nxt0 = cell(load_own(1));
     Raw nA{}, nB{}, hA{}, hB{};
-    if (2 <= nz) nA = load_at(Bown, 2, plane);
-    if (3 <= nz) nB = load_at(Bown, 3, plane);
+    if (2 <= nz) nA = load_own(2);
+    if (3 <= nz) nB = load_own(3);
     put_vel(ring[0], pown, prev);
     put_vel(ring[1], pown, cur0); put_sca(sS[0], pown, cur0);
     put_vel(ring[2], pown, nxt0); put_sca(sS[1], pown, nxt0);
     if (tid == 0) { sBad[0] = 0; sBad[1] = 0; }
     if (has_halo)
     {
-        put_vel(ring[0], phalo, cell(load_at(Bhalo, -1, hplane)));
-        const CellV h0 = cell(load_at(Bhalo, 0, hplane)), h1 = cell(load_at(Bhalo, 1, hplane));
+        put_vel(ring[0], phalo, cell(load_halo(-1)));
+        const CellV h0 = cell(load_halo(0)), h1 = cell(load_halo(1));
         put_vel(ring[1], phalo, h0); put_sca(sS[0], phalo, h0);
         put_vel(ring[2], phalo, h1); put_sca(sS[1], phalo, h1);
-        if (2 <= nz) hA = load_at(Bhalo, 2, hplane);
-        if (3 <= nz) hB = load_at(Bhalo, 3, hplane);
+        if (2 <= nz) hA = load_halo(2);
+        if (3 <= nz) hB = load_halo(3);
     }
     lds_barrier();
 
@@ -483,6 +533,9 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     // Returns false when the block must be left to the fix-up kernel.
     auto plane_step = [&](int k, Raw& nr, Raw& hr) -> bool
     {
+        // the x pushes staged by plane k-1 (visible since its last barrier;
+        // the staging buffer is refilled only after this plane's first barrier)
+        if (STG && A.push_ghosts && k > 0) flush_x(G, A.push, A.qout, xs, lb, k - 1);
         const double* sm = ring[0];
         const double* s0 = ring[1];
         const double* sp = ring[2];
@@ -609,6 +662,7 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             const double sc = S::keep ? 0.125 : 1.0;
             fx = sc*(jc*ix); fy = sc*(jc*iy); fz = sc*(jc*iz);
         }
+        double rr[5];
         #pragma unroll
         for (int n = 0; n < 5; ++n)
         {
@@ -631,8 +685,45 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
                 r += zlo[n];
                 r -= zhi[n];
             }
-            __builtin_nontemporal_store(r, ro + n*sv);    // streamed: nothing in this launch reads it again
+            // streamed: nothing in this launch reads it again
+            if (!STG || A.store_rhs) __builtin_nontemporal_store(r, ro + n*sv);
+            rr[n] = r;
             zlo[n] = zhi[n];
+        }
+        if constexpr (STG)
+        {
+            // the stage of cell (i,j,k): its stored primitives are the window's
+            const long long co = own_off + k*plane;
+            double qq[5] = {cur.p, cur.T, cur.u, cur.v, cur.w}, cons[5];
+            p2c(A.gamma, A.R, qq, cons);
+            #pragma unroll
+            for (int t = 0; t < 2; ++t)
+            {
+                if (t + 1 >= A.n || A.c[t] == 0.0) continue;
+                double inc[5];
+                #pragma unroll
+                for (int v = 0; v < 5; ++v) inc[v] = A.c[t]*A.r[t][co + v*sv];
+                #pragma unroll
+                for (int v = 0; v < 5; ++v) cons[v] += inc[v];
+            }
+            const double cl = A.c[A.n - 1];
+            if (cl != 0.0)
+            {
+                double inc[5];
+                #pragma unroll
+                for (int v = 0; v < 5; ++v) inc[v] = cl*rr[v];
+                #pragma unroll
+                for (int v = 0; v < 5; ++v) cons[v] += inc[v];
+            }
+            c2p(A.gamma, A.R, cons, qq);
+            #pragma unroll
+            for (int v = 0; v < 5; ++v) __builtin_nontemporal_store(qq[v], A.qout + co + v*sv);
+            if (A.push_ghosts)
+            {
+                const CellIt c{i, j, k, lb};
+                push_ghosts(G, A.push, A.qout, c, qq, true);
+                stage_x(G, xs, c, qq);
+            }
         }
 
         if (k + 1 < nz)
@@ -649,12 +740,12 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             }
             const CellV c2 = cell(nr);
             put_vel(s_new, pown, c2); put_sca(t_new, pown, c2);
-            if (k + 4 <= nz) nr = load_at(Bown, k + 4, plane);
+            if (k + 4 <= nz) nr = load_own(k + 4);
             if (has_halo)
             {
                 const CellV h = cell(hr);
                 put_vel(s_new, phalo, h); put_sca(t_new, phalo, h);
-                if (k + 4 <= nz) hr = load_at(Bhalo, k + 4, hplane);
+                if (k + 4 <= nz) hr = load_halo(k + 4);
             }
         }
         lds_barrier();
@@ -666,6 +757,7 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
         if (!plane_step(k, nA, hA)) return;
         if (k + 1 < nz && !plane_step(k + 1, nB, hB)) return;
     }
+    if (STG && A.push_ghosts) flush_x(G, A.push, A.qout, xs, lb, nz - 1);   // after the last plane's closing barrier
     if (tid == 0) start[lb] = nz;
 }
 
@@ -675,16 +767,16 @@ void launch_fast(const Grid& g, const double* prims, double* rhs, const FastCfg&
     const bool map = !g.geom.identity;
     if (increment && map)
         hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, true, true>), dim3(g.geom.nblk), dim3(256), 0, s,
-                           g.geom, prims, rhs, C, g.d_start);
+                           g.geom, prims, rhs, C, g.d_start, StageArgs{});
     else if (increment)
         hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, true, false>), dim3(g.geom.nblk), dim3(256), 0, s,
-                           g.geom, prims, rhs, C, g.d_start);
+                           g.geom, prims, rhs, C, g.d_start, StageArgs{});
     else if (map)
         hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, true>), dim3(g.geom.nblk), dim3(256), 0, s,
-                           g.geom, prims, rhs, C, g.d_start);
+                           g.geom, prims, rhs, C, g.d_start, StageArgs{});
     else
         hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, false>), dim3(g.geom.nblk), dim3(256), 0, s,
-                           g.geom, prims, rhs, C, g.d_start);
+                           g.geom, prims, rhs, C, g.d_start, StageArgs{});
     SPD_LAUNCH_CHECK();
 }
 
@@ -696,6 +788,35 @@ bool dispatch_visc(int visc, const Grid& g, const double* prims, double* rhs, co
         case 0: if (KEEP || PDISS) { launch_fast<KEEP, PDISS, 0>(g, prims, rhs, C, inc, s); return true; } return false;
         case 1: launch_fast<KEEP, PDISS, 1>(g, prims, rhs, C, inc, s); return true;
         case 2: launch_fast<KEEP, PDISS, 2>(g, prims, rhs, C, inc, s); return true;
+    }
+    return false;
+}
+
+template <int KEEP, int PDISS, int VISC>
+void launch_fast_stage(const Grid& g, const double* qin, double* rhs, const FastCfg& C, const StageArgs& A,
+                       bool read_neighbours, hipStream_t s)
+{
+    if (!g.geom.identity)
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, true, 1>), dim3(g.geom.nblk), dim3(256), 0, s,
+                           g.geom, qin, rhs, C, g.d_start, A);
+    else if (read_neighbours)
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, false, 2>), dim3(g.geom.nblk), dim3(256), 0, s,
+                           g.geom, qin, rhs, C, g.d_start, A);
+    else
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, false, 1>), dim3(g.geom.nblk), dim3(256), 0, s,
+                           g.geom, qin, rhs, C, g.d_start, A);
+    SPD_LAUNCH_CHECK();
+}
+
+template <int KEEP, int PDISS>
+bool dispatch_visc_stage(int visc, const Grid& g, const double* qin, double* rhs, const FastCfg& C,
+                         const StageArgs& A, bool nb, hipStream_t s)
+{
+    switch (visc)
+    {
+        case 0: if (KEEP || PDISS) { launch_fast_stage<KEEP, PDISS, 0>(g, qin, rhs, C, A, nb, s); return true; } return false;
+        case 1: launch_fast_stage<KEEP, PDISS, 1>(g, qin, rhs, C, A, nb, s); return true;
+        case 2: launch_fast_stage<KEEP, PDISS, 2>(g, qin, rhs, C, A, nb, s); return true;
     }
     return false;
 }
@@ -781,6 +902,75 @@ bool launch_flux_div_fast(const Grid& g, const double* prims, double* rhs, const
     // are completed by the per-cell kernel (an early exit per block when none)
     if (ok && vs == 1 + SPD_LAW_POWER) launch_flux_div_fixup(g, prims, rhs, f, increment, s);
     return ok;
+}
+
+// The fused stage needs the x-staged push (16 x 16 cross-sections, ng <= 2,
+// nz >= 2 ng) beside the fast kernel's own conditions
+static bool stage_geometry(const DevGeom& G)
+{
+    return G.nc[0] == 16 && G.nc[1] == 16 && G.ng >= 1 && G.ng <= 2 && G.nc[2] >= 2*G.ng;
+}
+
+bool stage_reads_neighbours_ok(const Grid& g)
+{
+    return g.geom.identity && g.desc.periodic[0] && g.desc.periodic[1] && g.desc.periodic[2];
+}
+
+bool flux_stage_fast_eligible(const Grid& g, const spd_flux_desc& f)
+{
+    int keep, pd, vs;
+    FastCfg C;
+    if (!stage_geometry(g.geom) || !select_fast(g.geom, f, keep, pd, vs, C)) return false;
+    return keep || pd || vs;
+}
+
+bool launch_flux_stage_fast(const Grid& g, const double* qin, double* qout, double* rhs, const spd_flux_desc& f,
+                            const double* const* rprev, const double* coef, int nr, bool store_rhs,
+                            double gamma, double R, hipStream_t s, bool read_neighbours, bool push)
+{
+    int keep, pd, vs;
+    FastCfg C;
+    if (!stage_geometry(g.geom) || !select_fast(g.geom, f, keep, pd, vs, C)) return false;
+    if (nr < 1 || nr > 3) throw ArgError{"a fused stage combines 1..3 residuals"};
+    if (qin == qout) throw ArgError{"a fused stage needs distinct input and output arrays"};
+    if (read_neighbours && !stage_reads_neighbours_ok(g))
+        throw ArgError{"neighbour-read stages need identity coordinates and every axis periodic"};
+    StageArgs A{};
+    A.qout = qout;
+    for (int t = 0; t + 1 < nr; ++t) A.r[t] = rprev[t];
+    for (int t = 0; t < nr; ++t) A.c[t] = coef[t];
+    A.n = nr;
+    A.store_rhs = store_rhs ? 1 : 0;
+    A.gamma = gamma; A.R = R;
+    A.push = g.d_push;
+    A.push_ghosts = push ? 1 : 0;
+    A.nbr = g.d_nbr;
+    bool ok = false;
+    switch (keep*3 + pd)
+    {
+        case 0: ok = dispatch_visc_stage<0, 0>(vs, g, qin, rhs, C, A, read_neighbours, s); break;
+        case 1: ok = dispatch_visc_stage<0, 1>(vs, g, qin, rhs, C, A, read_neighbours, s); break;
+        case 2: ok = dispatch_visc_stage<0, 2>(vs, g, qin, rhs, C, A, read_neighbours, s); break;
+        case 3: ok = dispatch_visc_stage<1, 0>(vs, g, qin, rhs, C, A, read_neighbours, s); break;
+        case 4: ok = dispatch_visc_stage<1, 1>(vs, g, qin, rhs, C, A, read_neighbours, s); break;
+        case 5: ok = dispatch_visc_stage<1, 2>(vs, g, qin, rhs, C, A, read_neighbours, s); break;
+    }
+    if (!ok) return false;
+    // power law: blocks with a face temperature outside the pow series range
+    // were left from that plane on; their residual (into rhs) and stage
+    // (qin -> qout) are completed here (early exits per block when none)
+    if (vs == 1 + SPD_LAW_POWER)
+    {
+        // the per-cell fix-up reads its block's ghost cells: after a
+        // neighbour-read stage they may be stale, so the flagged blocks pull
+        // theirs first (what an exchange copies; the others exit at once)
+        if (read_neighbours) launch_exchange_flagged(g, const_cast<double*>(qin), s);
+        launch_flux_div_fixup(g, qin, rhs, f, false, s);
+        const double* r[3] = {rprev && nr > 1 ? rprev[0] : nullptr, rprev && nr > 2 ? rprev[1] : nullptr, nullptr};
+        r[nr - 1] = rhs;
+        launch_stage_fixup(g, qin, qout, r, coef, nr, gamma, R, s);
+    }
+    return true;
 }
 
 } // namespace spd

@@ -40,9 +40,12 @@ __host__ __device__ inline int exchange_items(const DevGeom& G)
 }
 
 __global__ void __launch_bounds__(256)
-k_exchange(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a)
+k_exchange(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a, const int* __restrict__ only)
 {
     const int lb = blockIdx.y;
+    // only != null: blocks the LDS-staged flux kernel left to the fix-up
+    // (only[lb] < nz), the others exit at once
+    if (only && only[lb] >= G.nc[2]) return;
     const int t = blockIdx.x*blockDim.x + threadIdx.x;
     const int n0 = G.nc[0], n1 = G.nc[1], n2 = G.nc[2], ng = G.ng;
     const int N1 = G.N[1];
@@ -380,7 +383,16 @@ k_fill_affine(DevGeom G, double* __restrict__ a, int idir, int pm, spd_fill_kern
 void launch_exchange(const Grid& g, double* a, hipStream_t s)
 {
     const int items = exchange_items(g.geom);
-    hipLaunchKernelGGL(k_exchange, dim3((items + 255)/256, g.geom.nblk), dim3(256), 0, s, g.geom, g.d_nbr, a);
+    hipLaunchKernelGGL(k_exchange, dim3((items + 255)/256, g.geom.nblk), dim3(256), 0, s, g.geom, g.d_nbr, a,
+                       (const int*)nullptr);
+    SPD_LAUNCH_CHECK();
+}
+
+void launch_exchange_flagged(const Grid& g, double* a, hipStream_t s)
+{
+    const int items = exchange_items(g.geom);
+    hipLaunchKernelGGL(k_exchange, dim3((items + 255)/256, g.geom.nblk), dim3(256), 0, s, g.geom, g.d_nbr, a,
+                       (const int*)g.d_start);
     SPD_LAUNCH_CHECK();
 }
 

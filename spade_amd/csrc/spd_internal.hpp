@@ -108,11 +108,30 @@ void launch_flux_div_fixup(const Grid& g, const double* prims, double* rhs, cons
 bool launch_flux_div_fast(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f,
                           bool increment, hipStream_t s);
 bool flux_div_fast_eligible(const Grid& g, const spd_flux_desc& f);
+// One transform-aware RK stage fused with flux_div on the fast kernel:
+// qout <- c2p(p2c(qin) + sum_{j < nr-1} coef[j] r[j] + coef[nr-1] L(qin)) on
+// interior cells; L(qin) is also written to rhs when store_rhs (and, for the
+// planes left to the fix-up, always).  push: the ghosts the result feeds are
+// written into qout (else qout's ghost cells are left alone).
+// read_neighbours: L(qin) takes its halo from the neighbour blocks' interiors
+// instead of qin's ghost cells (identity coordinates, every axis periodic:
+// stage_reads_neighbours_ok), so qin's ghosts need not be current.
+// Bit-identical to run_flux_div into rhs followed by launch_rk_transform_stage
+// on qin (whose ghosts hold what an exchange copies).  Needs a fast-kernel
+// composition and 16 x 16 x nz blocks (nz >= 2 ng, ng <= 2); false when not
+// eligible (nothing launched).  qin and qout must be distinct arrays.
+bool launch_flux_stage_fast(const Grid& g, const double* qin, double* qout, double* rhs, const spd_flux_desc& f,
+                            const double* const* rprev, const double* coef, int nr, bool store_rhs,
+                            double gamma, double R, hipStream_t s, bool read_neighbours, bool push);
+bool flux_stage_fast_eligible(const Grid& g, const spd_flux_desc& f);
+bool stage_reads_neighbours_ok(const Grid& g);
 // non-zero when the composition has a compile-time specialised per-cell kernel
 int flux_div_spec_code(const spd_flux_desc& f);
 // the composition contains a hybrid_scheme_t term
 bool flux_has_hybrid(const spd_flux_desc& f);
 void launch_exchange(const Grid& g, double* a, hipStream_t s);
+// the exchange of the blocks with d_start[lb] < nz only (the others exit at once)
+void launch_exchange_flagged(const Grid& g, double* a, hipStream_t s);
 void launch_wall_fill(const Grid& g, double* a, int face, double t_wall, hipStream_t s);
 // boundary_fill over the faces of face_mask (bit 2*dir+pm), one launch per face in face order
 void launch_fill_extrap(const Grid& g, double* a, int face_mask, int order, hipStream_t s);
@@ -126,6 +145,11 @@ void launch_soa_to_aos(const Grid& g, const double* a, double* stage, int lb0, i
 // push = false leaves every ghost cell alone (interior update only)
 void launch_rk_transform_stage(const Grid& g, double* q, const double* const* r, const double* coef, int nr,
                                double gamma, double R, hipStream_t s, bool push = true);
+// the same stage, qin -> qout (ghosts pushed into qout), for the planes
+// k >= d_start[lb] of each block only (the cells a fused launch left to the
+// fix-up); 16 x 16 block cross-sections
+void launch_stage_fixup(const Grid& g, const double* qin, double* qout, const double* const* r, const double* coef,
+                        int nr, double gamma, double R, hipStream_t s);
 // grid_array element-wise operations: scale every stored double; a (+|-)= b on
 // interior cells; convert_state in place (every stored cell or the interior)
 void launch_array_scale(const Grid& g, double* a, double sc, hipStream_t s);
