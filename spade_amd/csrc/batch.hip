@@ -134,7 +134,11 @@ __device__ inline PrimD to_prim(const CaseParams& P, const Cons& c)
 {
     PrimD q;
     q.rho = c.r;
-    const double inv = 1.0/c.r;
+    // 1/rho: hardware reciprocal and two Newton steps (within an ulp or so;
+    // this engine's contract is the FP32-flux tolerance, not bit parity)
+    double inv = __builtin_amdgcn_rcp(c.r);
+    inv = __builtin_fma(inv, __builtin_fma(-c.r, inv, 1.0), inv);
+    inv = __builtin_fma(inv, __builtin_fma(-c.r, inv, 1.0), inv);
     q.u = c.mu*inv; q.v = c.mv*inv; q.w = c.mw*inv;
     q.p = P.gm1*(c.E - 0.5*c.r*(q.u*q.u + q.v*q.v + q.w*q.w));
     q.T = q.p*inv*P.inv_R;
@@ -205,8 +209,8 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
     __shared__ float  sU[3][3*HP];        // u, v, w (f32) of planes k-1, k, k+1
     __shared__ float  sS[2][4*HP];        // p', T', rho', e' of planes k, k+1 (parity)
     __shared__ double sV[2][3*HP];        // u, v, w (f64) of planes k, k+1 (parity)
-    __shared__ float  sFx[5][TJ*17];
-    __shared__ float  sFy[5][(TJ + 1)*16];
+    __shared__ double sFx[5][TJ*17];      // face fluxes, converted to FP64 once by the face's owner
+    __shared__ double sFy[5][(TJ + 1)*16];
     __shared__ double sRed[2][T::NT];
 
     const int tid = threadIdx.x;
@@ -305,11 +309,13 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
         double wprev_u = qprev.u, wprev_v = qprev.v, wprev_w = qprev.w;   // own column, plane k-1 (f64)
         lds_barrier();
 
-        float zlo[5];
+        double zlo[5];                   // low z-face flux of the current plane (FP64 copy)
         {
-            float g[3][3], gT;
+            float g[3][3], gT, F[5];
             gradsf_z<HP>(sU[slotk(-1)], sU[slotk(0)], pown, prev, cur, ix, iy, iz, g, gT);
-            face_f32<2, LAW>(P, prev, cur, g, gT, zlo);
+            face_f32<2, LAW>(P, prev, cur, g, gT, F);
+            #pragma unroll
+            for (int n = 0; n < 5; ++n) zlo[n] = (double)F[n];
         }
 
         for (int k = 0; k < 16; ++k)
@@ -325,7 +331,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 float g[3][3], gT, F[5];
                 gradsf_x<HP>(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
                 face_f32<0, LAW>(P, L, cur, g, gT, F);
-                for (int n = 0; n < 5; ++n) sFx[n][j*17 + i] = F[n];
+                for (int n = 0; n < 5; ++n) sFx[n][j*17 + i] = (double)F[n];
             }
             {
                 const int pL = pown - HXB;
@@ -333,7 +339,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 float g[3][3], gT, F[5];
                 gradsf_y<HP>(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
                 face_f32<1, LAW>(P, L, cur, g, gT, F);
-                for (int n = 0; n < 5; ++n) sFy[n][j*16 + i] = F[n];
+                for (int n = 0; n < 5; ++n) sFy[n][j*16 + i] = (double)F[n];
             }
             if (tid >= T::XB && tid < T::XB + TJ)
             {
@@ -343,7 +349,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 float g[3][3], gT, F[5];
                 gradsf_x<HP>(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
                 face_f32<0, LAW>(P, L, Rr, g, gT, F);
-                for (int n = 0; n < 5; ++n) sFx[n][jj*17 + 16] = F[n];
+                for (int n = 0; n < 5; ++n) sFx[n][jj*17 + 16] = (double)F[n];
             }
             if (tid >= T::YB && tid < T::YB + 16)
             {
@@ -353,7 +359,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 float g[3][3], gT, F[5];
                 gradsf_y<HP>(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
                 face_f32<1, LAW>(P, L, Rr, g, gT, F);
-                for (int n = 0; n < 5; ++n) sFy[n][TJ*16 + ii] = F[n];
+                for (int n = 0; n < 5; ++n) sFy[n][TJ*16 + ii] = (double)F[n];
             }
             float zhi[5];
             {
@@ -367,7 +373,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             const double dvy = vk[HP + pown + HXB] - vk[HP + pown - HXB];          // v(j+1) - v(j-1)
             const double* vn = sV[(k + 1) & 1];
             const double dwz = vn[2*HP + pown] - wprev_w;                          // w(k+1) - w(k-1)
-            const double DU = -0.5*(dux*P.ix + dvy*P.iy + dwz*P.iz);               // -div u
+            const double DU = -0.5*__builtin_fma(dux, P.ix, __builtin_fma(dvy, P.iy, dwz*P.iz));   // -div u
             if (want_diag)
             {
                 const double dwy = vk[2*HP + pown + HXB] - vk[2*HP + pown - HXB];
@@ -386,25 +392,27 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
 
             // FP64 residual sum and stage combination for cell (i,j,k); the
             // difference of two FP32 face fluxes is exact in FP64, so each
-            // direction is differenced first and scaled once
+            // direction is differenced first and scaled once (fused
+            // multiply-adds written out, the same in every launch shape)
             double r[5];
             #pragma unroll
             for (int n = 0; n < 5; ++n)
             {
-                r[n] = ((double)sFx[n][j*17 + i] - (double)sFx[n][j*17 + i + 1])*P.ix
-                     + ((double)sFy[n][j*16 + i] - (double)sFy[n][(j + 1)*16 + i])*P.iy
-                     + ((double)zlo[n] - (double)zhi[n])*P.iz;
-                zlo[n] = zhi[n];
+                const double zh = (double)zhi[n];
+                r[n] = __builtin_fma(sFx[n][j*17 + i] - sFx[n][j*17 + i + 1], P.ix,
+                       __builtin_fma(sFy[n][j*16 + i] - sFy[n][(j + 1)*16 + i], P.iy, (zlo[n] - zh)*P.iz));
+                zlo[n] = zh;
             }
-            const double d_rho = r[0] + P.rho_ref*DU;
-            const double d_E   = r[1] + P.e_ref*d_rho + P.p_ref*DU;
+            const double d_rho = __builtin_fma(P.rho_ref, DU, r[0]);
+            const double d_E   = __builtin_fma(P.p_ref, DU, __builtin_fma(P.e_ref, d_rho, r[1]));
             const double res[5] = {d_rho, d_E, r[2], r[3], r[4]};
             #pragma unroll
             for (int n = 0; n < 5; ++n)
             {
                 const double uin = n == 0 ? rcur.r : n == 1 ? rcur.E : n == 2 ? rcur.mu : n == 3 ? rcur.mv : rcur.mw;
-                const double stage = uin + P.dt*res[n];
-                __builtin_nontemporal_store(HAS_U0 ? alpha*u0[n] + beta*stage : beta*stage, Qout + o + n*4096);
+                const double stage = __builtin_fma(P.dt, res[n], uin);
+                __builtin_nontemporal_store(HAS_U0 ? __builtin_fma(alpha, u0[n], beta*stage) : beta*stage,
+                                            Qout + o + n*4096);
             }
 
             // advance the window
