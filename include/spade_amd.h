@@ -102,6 +102,7 @@ enum {
 enum { SPD_SENSOR_CONST = 0, SPD_SENSOR_DUCROS = 1 };   /* state_sensor.h:21-60 */
 enum { SPD_SPLIT_RUSANOV = 0, SPD_SPLIT_PHYS = 1 };     /* flux_funcs.h:8-126 */
 enum { SPD_LAW_CONST = 0, SPD_LAW_POWER = 1 };          /* viscous_laws.h:62-135 */
+enum { SPD_SGS_NONE = 0, SPD_SGS_WALE = 1 };            /* subgrid_scale.h:24-86 */
 #define SPD_MAX_TERMS 4
 
 typedef struct {
@@ -119,6 +120,10 @@ typedef struct {
                                         sensor / sensor_param = the blending sensor */
     int    full;                     /* HYBRID: 1 = full_flux ((1 - b) F_central + b F_diss),
                                         0 = diss_flux (F_central + b F_diss) (hybrid_scheme.h:7-12, 36-41) */
+    int    sgs;                      /* VISC: SPD_SGS_WALE = sgs_visc_t(law, wale_t(gas, sgs_cw, sgs_delta,
+                                        sgs_prt)) (viscous_laws.h:174-216, subgrid_scale.h:24-86): mu + mu_t,
+                                        beta - 0.66666666667 mu_t, alpha + mu_t / sgs_prt */
+    double sgs_cw, sgs_delta, sgs_prt;
 } spd_flux_term;
 
 typedef struct {

@@ -206,6 +206,18 @@ inline flux visc_lr(const power_law_t& law, const ideal_gas_t& gas)
     t.power = law.power; t.prandtl = law.prandtl; return flux(t, gas);
 }
 
+// subgrid_scale.h:24-86 and viscous_laws.h:174-216: a laminar law plus the
+// WALE subgrid viscosity (the gas of the flux gives R)
+struct wale_t { double cw, delta, prt; };
+template <typename laminar_t> struct sgs_visc_t { laminar_t lam; wale_t turb; };
+template <typename laminar_t> inline flux visc_lr(const sgs_visc_t<laminar_t>& law, const ideal_gas_t& gas)
+{
+    flux f = visc_lr(law.lam, gas);
+    spd_flux_term t = f.desc().terms[0];
+    t.sgs = SPD_SGS_WALE; t.sgs_cw = law.turb.cw; t.sgs_delta = law.turb.delta; t.sgs_prt = law.turb.prt;
+    return flux(t, gas);
+}
+
 // pde_algs::flux_div (pde-algs/flux-div/flux_div.h:16-56); default increment as in the reference
 inline void flux_div(const grid_array& prims, grid_array& rhs, const flux& f, bool increment = true)
 {

@@ -18,6 +18,7 @@ TERM_KEEP, TERM_TOTANI, TERM_PDISS, TERM_WENO, TERM_FWENO, TERM_FIRST, TERM_VISC
 SENSOR_CONST, SENSOR_DUCROS = 0, 1
 SPLIT_RUSANOV, SPLIT_PHYS = 0, 1
 LAW_CONST, LAW_POWER = 0, 1
+SGS_NONE, SGS_WALE = 0, 1
 MAX_TERMS = 4
 
 
@@ -40,7 +41,8 @@ class Term(C.Structure):
                 ("sensor_param", C.c_double), ("eps_p", C.c_double), ("eps_T", C.c_double),
                 ("split", C.c_int), ("smooth", C.c_int), ("law", C.c_int),
                 ("mu", C.c_double), ("T_ref", C.c_double), ("power", C.c_double),
-                ("prandtl", C.c_double), ("diss_kind", C.c_int), ("full", C.c_int)]
+                ("prandtl", C.c_double), ("diss_kind", C.c_int), ("full", C.c_int),
+                ("sgs", C.c_int), ("sgs_cw", C.c_double), ("sgs_delta", C.c_double), ("sgs_prt", C.c_double)]
 
 
 class Flux(C.Structure):

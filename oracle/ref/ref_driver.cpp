@@ -264,6 +264,16 @@ int main(int argc, char** argv)
         run("weno_fds", spade::convective::weno_fds_t<decltype(rus_fds)>(rus_fds));
         run("weno_fds_lin", spade::convective::weno_fds_t<decltype(rus_fds), spade::convective::disable_smooth>(rus_fds));
         run("keep2_pdiss_visc_const", spade::omni::compose(spade::convective::cent_keep<2>(air), pd_ducros, vsc_const));
+        // sgs_visc_t: a laminar law plus the WALE subgrid viscosity
+        // (viscous_laws.h:174-216, subgrid_scale.h:24-86), C_w 0.5, filter
+        // width 0.3, turbulent Prandtl number 0.9
+        spade::subgrid_scale::wale_t<real_t, decltype(air)> wale(air, 0.5, 0.3, 0.9);
+        spade::viscous_laws::sgs_visc_t sgs_const(visc_const, wale);
+        spade::viscous_laws::sgs_visc_t sgs_pow(visc_pow, wale);
+        spade::viscous::visc_lr vsc_sgs_const(sgs_const, air);
+        spade::viscous::visc_lr vsc_sgs_pow(sgs_pow, air);
+        run("visc_sgs_const", vsc_sgs_const);
+        run("keep2_visc_sgs_pow", spade::omni::compose(spade::convective::cent_keep<2>(air), vsc_sgs_pow));
         // central / WENO blends (navier-stokes/hybrid_scheme.h)
         run("hybrid_keep2_weno_ducros_full",
             spade::convective::hybrid_scheme_t(spade::convective::cent_keep<2>(air), spade::convective::weno_t<decltype(rus)>(rus),

@@ -575,8 +575,46 @@ void f_first(const Ctx& C, const Face& f, double out[5])
     for (int n = 0; n < 5; ++n) out[n] = fuL[n] + fdR[n];
 }
 
-// viscous law: mu, beta, alpha (viscous_laws.h:81-84, 121-134)
-void visc_law(const ora_term& t, const Q& qf, double& mu, double& beta, double& alpha)
+// WALE subgrid viscosity at a face (subgrid_scale.h:44-84, after Nicoud and
+// Ducros 1999): with g_ij = d u_i / d x_j from the face gradient,
+//   S_ij = (g_ij + g_ji)/2,  Sd_ij = ((g^2)_ij + (g^2)_ji)/2 - delta_ij tr(g^2)/3,
+//   mu_t = rho cw^2 delta^2 |Sd|^2 |Sd| / (eps + |S|^2 |S|^2 |S| + |Sd|^2 |Sd|^(1/2))
+// with |A|^2 = sum A_ij^2, eps = 1e-8, rho = p/(R T) of the face value; every
+// sum in the reference's order (columns outer, rows inner for |A|^2).
+double wale_mu_t(const ora_term& t, const Q& qf, const double g[3][5], double rgas)
+{
+    const double rho = qf.p/(rgas*qf.T);
+    auto gij = [&](int i, int j) { return g[j][2 + i]; };
+    double g2[3][3], sd[3][3], s[3][3];
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i)
+            g2[i][j] = gij(i, 0)*gij(0, j) + gij(i, 1)*gij(1, j) + gij(i, 2)*gij(2, j);
+    for (int j = 0; j < 3; ++j)
+        for (int i = 0; i < 3; ++i)
+        {
+            sd[i][j] = 0.5*(g2[i][j] + g2[j][i]);
+            s[i][j]  = 0.5*(gij(i, j) + gij(j, i));
+        }
+    double tr = 0.0;
+    for (int d = 0; d < 3; ++d) tr += 0.33333333333333333*g2[d][d];
+    for (int d = 0; d < 3; ++d) sd[d][d] -= tr;
+    auto sq = [](const double a[3][3])
+    {
+        double acc = 0.0;
+        for (int j = 0; j < 3; ++j)
+            for (int i = 0; i < 3; ++i) acc += a[i][j]*a[i][j];
+        return acc;
+    };
+    const double ssd = sq(sd), ss = sq(s);
+    const double r0 = std::sqrt(ssd), r1 = std::sqrt(r0), r2 = std::sqrt(ss);
+    return rho*t.sgs_cw*t.sgs_cw*t.sgs_delta*t.sgs_delta*ssd*r0/(1e-8 + ss*ss*r2 + ssd*r1);
+}
+
+// viscous law: mu, beta, alpha (viscous_laws.h:81-84, 121-134); with a
+// subgrid model, sgs_visc_t::get_all (viscous_laws.h:186-196) adds mu_t,
+// -0.66666666667 mu_t and mu_t / Pr_t
+void visc_law(const ora_term& t, const Q& qf, const double g[3][5], double rgas,
+              double& mu, double& beta, double& alpha)
 {
     if (t.law == ORA_LAW_POWER)
     {
@@ -590,6 +628,13 @@ void visc_law(const ora_term& t, const Q& qf, double& mu, double& beta, double& 
         beta  = double(-2.0*t.mu/3.0);
         alpha = t.mu*(1.0/t.prandtl);
     }
+    if (t.sgs == ORA_SGS_WALE)
+    {
+        const double mu_t = wale_mu_t(t, qf, g, rgas);
+        mu    = mu + mu_t;
+        beta  = beta - 0.66666666667*mu_t;
+        alpha = alpha + mu_t/t.sgs_prt;
+    }
 }
 
 // visc_lr (viscous.h:39-112)
@@ -599,7 +644,7 @@ void f_visc(const Ctx& C, const Face& f, const ora_term& t, double out[5])
     double g[3][5]; face_grad(C.G, C.a, f, g);
     double n[3]; face_metric(C.G, f, n);
     double visc, visc2, alpha;
-    visc_law(t, qf, visc, visc2, alpha);
+    visc_law(t, qf, g, C.gas.R, visc, visc2, alpha);
     enum { P = 0, T = 1, U = 2, V = 3, W = 4 };
     const double div = g[0][U] + g[1][V] + g[2][W];
     double tau[3][3];

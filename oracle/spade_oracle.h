@@ -32,6 +32,7 @@ enum { ORA_TERM_KEEP = 1, ORA_TERM_TOTANI, ORA_TERM_PDISS, ORA_TERM_WENO,
 enum { ORA_SENSOR_CONST = 0, ORA_SENSOR_DUCROS = 1 };
 enum { ORA_SPLIT_RUSANOV = 0, ORA_SPLIT_PHYS = 1 };
 enum { ORA_LAW_CONST = 0, ORA_LAW_POWER = 1 };
+enum { ORA_SGS_NONE = 0, ORA_SGS_WALE = 1 };
 #define ORA_MAX_TERMS 4
 
 typedef struct
@@ -47,6 +48,8 @@ typedef struct
     double mu, T_ref, power, prandtl;
     int diss_kind;              /* HYBRID: dissipative scheme, ORA_TERM_WENO or ORA_TERM_FWENO */
     int full;                   /* HYBRID: 1 = (1-b) central + b dissipative, 0 = central + b dissipative */
+    int sgs;                    /* VISC: ORA_SGS_WALE = sgs_visc_t(law, wale_t(cw, delta, prt)) */
+    double sgs_cw, sgs_delta, sgs_prt;
 } ora_term;
 
 typedef struct { int nterms; ora_term terms[ORA_MAX_TERMS]; } ora_flux;

@@ -336,6 +336,18 @@ def power_law(mu_ref, T_ref, power, prandtl):
                 prandtl=float(prandtl))
 
 
+def wale(gas, cw, delta, prt):
+    """wale_t(gas, cw, delta, prt) (subgrid_scale.h:24-86): the WALE subgrid
+    viscosity with coefficient cw, filter width delta and turbulent Prandtl
+    number prt; combine with a laminar law through sgs_visc."""
+    return dict(sgs=_abi.SGS_WALE, sgs_cw=float(cw), sgs_delta=float(delta), sgs_prt=float(prt))
+
+
+def sgs_visc(law, model):
+    """sgs_visc_t(laminar law, subgrid model) (viscous_laws.h:174-216)."""
+    return _Law(**law.fields, **model)
+
+
 def visc_lr(law, gas):
     return _Term(gas, kind=_abi.TERM_VISC, **law.fields)
 

@@ -53,6 +53,9 @@ void validate_flux(const spd_flux_desc* f, int ng)
                 require(t.law == SPD_LAW_CONST || t.law == SPD_LAW_POWER, "viscous law must be SPD_LAW_CONST or SPD_LAW_POWER");
                 require(t.prandtl > 0.0, "viscous law needs a Prandtl number > 0");
                 require(t.law == SPD_LAW_CONST || t.T_ref > 0.0, "power law needs T_ref > 0");
+                require(t.sgs == SPD_SGS_NONE || t.sgs == SPD_SGS_WALE, "subgrid model must be SPD_SGS_NONE or SPD_SGS_WALE");
+                require(t.sgs == SPD_SGS_NONE || (t.sgs_prt > 0.0 && std::isfinite(t.sgs_cw) && std::isfinite(t.sgs_delta)),
+                        "WALE needs a turbulent Prandtl number > 0 and finite cw, delta");
                 break;
             case SPD_TERM_TOTANI: case SPD_TERM_FIRST:
                 require(ng >= 1, "flux needs at least one ghost layer");

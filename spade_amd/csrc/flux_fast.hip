@@ -856,6 +856,7 @@ bool select_fast(const DevGeom& G, const spd_flux_desc& f, int& keep, int& pd, i
         }
         else if (t.kind == SPD_TERM_VISC && stage < 3)
         {
+            if (t.sgs != SPD_SGS_NONE) return false;   // the per-cell kernels evaluate subgrid models
             vs = 1 + t.law; stage = 3;
             // constant law (viscous_laws.h:77-99): beta = -2 mu / 3, alpha = mu / Pr
             C.mu_s = sc*t.mu;

@@ -478,8 +478,49 @@ __device__ __forceinline__ void f_first(const Cell& C, const FaceIdx& f, double 
     for (int n = 0; n < 5; ++n) out[n] = fuL[n] + fdR[n];
 }
 
+// WALE subgrid viscosity at a face (subgrid_scale.h:44-84): with
+// g_ij = d u_i / d x_j of the face gradient, S = (g + g^T)/2,
+// Sd = (g^2 + (g^2)^T)/2 - tr(g^2)/3 I, |A|^2 = sum of squares (columns outer),
+//   mu_t = rho cw^2 delta^2 |Sd|^2 |Sd| / (1e-8 + |S|^4 |S| + |Sd|^2 |Sd|^(1/2)),
+// rho = p / (R T) of the face value; each sum in the reference's order
+__device__ inline double wale_mu_t(const spd_flux_term& t, const Prim& qf, const double g[3][5], double R)
+{
+    const double rho = qf.p/(R*qf.T);
+    double g2[3][3], sd[3][3], s[3][3];
+    #pragma unroll
+    for (int j = 0; j < 3; ++j)
+        #pragma unroll
+        for (int i = 0; i < 3; ++i)
+            g2[i][j] = g[0][2 + i]*g[j][2] + g[1][2 + i]*g[j][3] + g[2][2 + i]*g[j][4];
+    #pragma unroll
+    for (int j = 0; j < 3; ++j)
+        #pragma unroll
+        for (int i = 0; i < 3; ++i)
+        {
+            sd[i][j] = 0.5*(g2[i][j] + g2[j][i]);
+            s[i][j]  = 0.5*(g[j][2 + i] + g[i][2 + j]);
+        }
+    double tr = 0.0;
+    #pragma unroll
+    for (int d = 0; d < 3; ++d) tr += 0.33333333333333333*g2[d][d];
+    #pragma unroll
+    for (int d = 0; d < 3; ++d) sd[d][d] -= tr;
+    double ssd = 0.0, ss = 0.0;
+    #pragma unroll
+    for (int j = 0; j < 3; ++j)
+        #pragma unroll
+        for (int i = 0; i < 3; ++i) ssd += sd[i][j]*sd[i][j];
+    #pragma unroll
+    for (int j = 0; j < 3; ++j)
+        #pragma unroll
+        for (int i = 0; i < 3; ++i) ss += s[i][j]*s[i][j];
+    const double r0 = sqrt(ssd), r1 = sqrt(r0), r2 = sqrt(ss);
+    return rho*t.sgs_cw*t.sgs_cw*t.sgs_delta*t.sgs_delta*ssd*r0/(1e-8 + ss*ss*r2 + ssd*r1);
+}
+
 // visc_lr (viscous.h:39-112) with constant_viscosity_t or the power law
-// (viscous_laws.h:62-135)
+// (viscous_laws.h:62-135), optionally inside sgs_visc_t with the WALE model
+// (viscous_laws.h:186-196)
 __device__ __forceinline__ void f_visc(const Cell& C, const FaceIdx& f, const spd_flux_term& t, double R, double gam, double out[5])
 {
     const Prim ql = cell_at(C, f, -1), qr = cell_at(C, f, 1);
@@ -501,6 +542,13 @@ __device__ __forceinline__ void f_visc(const Cell& C, const FaceIdx& f, const sp
         visc  = t.mu;
         visc2 = double(-2.0*t.mu/3.0);
         alpha = t.mu*(1.0/t.prandtl);
+    }
+    if (t.sgs == SPD_SGS_WALE)
+    {
+        const double mu_t = wale_mu_t(t, qf, g, R);
+        visc  = visc + mu_t;
+        visc2 = visc2 - 0.66666666667*mu_t;
+        alpha = alpha + mu_t/t.sgs_prt;
     }
     const double div = g[0][2] + g[1][3] + g[2][4];
     double tau[3][3];

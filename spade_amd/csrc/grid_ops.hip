@@ -39,14 +39,10 @@ __host__ __device__ inline int exchange_items(const DevGeom& G)
     return (2*G.ng*G.N[1] + 2*G.ng*G.nc[2])*G.nc[0] + 2*G.N[1]*G.N[2];
 }
 
-__global__ void __launch_bounds__(256)
-k_exchange(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a, const int* __restrict__ only)
+// Exchange item t of block lb (storage order of its ghost cells, see above)
+__device__ inline void exchange_item(const DevGeom& G, const int* __restrict__ nbr, double* __restrict__ a,
+                                     int lb, int t)
 {
-    const int lb = blockIdx.y;
-    // only != null: blocks the LDS-staged flux kernel left to the fix-up
-    // (only[lb] < nz), the others exit at once
-    if (only && only[lb] >= G.nc[2]) return;
-    const int t = blockIdx.x*blockDim.x + threadIdx.x;
     const int n0 = G.nc[0], n1 = G.nc[1], n2 = G.nc[2], ng = G.ng;
     const int N1 = G.N[1];
     const int rows = N1*G.N[2];
@@ -107,6 +103,23 @@ k_exchange(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a, const
         #pragma unroll
         for (int v = 0; v < 5; ++v) d[v*sv] = q[v];
     }
+}
+
+__global__ void __launch_bounds__(256)
+k_exchange(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a)
+{
+    exchange_item(G, nbr, a, blockIdx.y, blockIdx.x*blockDim.x + threadIdx.x);
+}
+
+// The exchange of the blocks the LDS-staged flux kernel left to the fix-up
+// (only[lb] < nz): one workgroup per block, the others exit at once
+__global__ void __launch_bounds__(256)
+k_exchange_flagged(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a, const int* __restrict__ only)
+{
+    const int lb = blockIdx.x;
+    if (only[lb] >= G.nc[2]) return;
+    const int items = exchange_items(G);
+    for (int t = threadIdx.x; t < items; t += blockDim.x) exchange_item(G, nbr, a, lb, t);
 }
 
 // boundary_fill with the isothermal no-slip mirror (grid/boundary_fill.h:
@@ -383,15 +396,13 @@ k_fill_affine(DevGeom G, double* __restrict__ a, int idir, int pm, spd_fill_kern
 void launch_exchange(const Grid& g, double* a, hipStream_t s)
 {
     const int items = exchange_items(g.geom);
-    hipLaunchKernelGGL(k_exchange, dim3((items + 255)/256, g.geom.nblk), dim3(256), 0, s, g.geom, g.d_nbr, a,
-                       (const int*)nullptr);
+    hipLaunchKernelGGL(k_exchange, dim3((items + 255)/256, g.geom.nblk), dim3(256), 0, s, g.geom, g.d_nbr, a);
     SPD_LAUNCH_CHECK();
 }
 
 void launch_exchange_flagged(const Grid& g, double* a, hipStream_t s)
 {
-    const int items = exchange_items(g.geom);
-    hipLaunchKernelGGL(k_exchange, dim3((items + 255)/256, g.geom.nblk), dim3(256), 0, s, g.geom, g.d_nbr, a,
+    hipLaunchKernelGGL(k_exchange_flagged, dim3(g.geom.nblk), dim3(256), 0, s, g.geom, g.d_nbr, a,
                        (const int*)g.d_start);
     SPD_LAUNCH_CHECK();
 }

@@ -12,6 +12,8 @@ VP = dict(kind=7, law=1, mu=P["mu_ref"], T_ref=P["T_ref"], power=P["power"], pra
 PDC = dict(kind=3, sensor=0, sensor_param=0.7, eps_p=0.3, eps_T=0.2)
 PDD = dict(kind=3, sensor=1, sensor_param=1e-3, eps_p=0.3, eps_T=0.2)
 K2 = dict(kind=1, order=2)
+# sgs_visc_t(law, wale_t(gas, cw, delta, prt)) (viscous_laws.h:174-216, subgrid_scale.h:24-86)
+WALE = dict(sgs=1, sgs_cw=0.5, sgs_delta=0.3, sgs_prt=0.9)
 K4 = dict(kind=1, order=4)
 
 FLUX_CASES = {
@@ -33,6 +35,9 @@ FLUX_CASES = {
     "weno_fds": [dict(kind=9, smooth=1)],
     "weno_fds_lin": [dict(kind=9, smooth=0)],
     "keep2_pdiss_visc_const": [K2, PDD, VC],
+    # visc_lr with a laminar law plus the WALE subgrid viscosity
+    "visc_sgs_const": [dict(VC, **WALE)],
+    "keep2_visc_sgs_pow": [K2, dict(VP, **WALE)],
     # hybrid_scheme_t: cent_keep<order> blended with WENO by a face sensor
     "hybrid_keep2_weno_ducros_full": [dict(kind=8, order=2, sensor=1, sensor_param=1e-3, split=0, smooth=1,
                                            diss_kind=4, full=1)],
@@ -44,7 +49,7 @@ FLUX_CASES = {
 
 # cases whose arithmetic involves pow(): equal to the reference within
 # round-off, not bit for bit, on the GPU (device pow vs glibc pow)
-USES_POW = {"visc_pow", "keep2_visc_pow"}
+USES_POW = {"visc_pow", "keep2_visc_pow", "keep2_visc_sgs_pow"}
 
 # compositions the CPU oracle reproduces bit for bit against the reference but
 # the GPU library refuses (SPD_ERR_UNSUPPORTED): its first hybrid_scheme_t

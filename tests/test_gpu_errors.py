@@ -57,6 +57,8 @@ def test_grid_blocks_narrower_than_ghost_layers():
     (dict(kind=7, law=2, mu=1e-3, prandtl=0.72), "viscous law"),
     (dict(kind=7, law=0, mu=1e-3, prandtl=0.0), "Prandtl"),
     (dict(kind=7, law=1, mu=1e-3, T_ref=0.0, power=0.7, prandtl=0.72), "T_ref"),
+    (dict(kind=7, law=0, mu=1e-3, prandtl=0.72, sgs=2), "subgrid model"),
+    (dict(kind=7, law=0, mu=1e-3, prandtl=0.72, sgs=1, sgs_cw=0.5, sgs_delta=0.1, sgs_prt=0.0), "turbulent Prandtl"),
     (dict(kind=3, sensor=5, sensor_param=0.1, eps_p=0.1, eps_T=0.1), "sensor"),
     (dict(kind=4, split=3, smooth=1), "split"),
 ])
