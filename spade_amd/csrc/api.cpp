@@ -288,14 +288,14 @@ int spd_grid_create(spd_ctx_t ctx, const spd_grid_desc* d, spd_grid_t* out)
                 if (nbr[27*D + e] >= 0) push[27*nbr[27*D + e] + e] = D;
         SPD_HIP(hipMalloc((void**)&g.d_nbr, sizeof(int)*27*dg.nblk));
         SPD_HIP(hipMalloc((void**)&g.d_push, sizeof(int)*27*dg.nblk));
-        SPD_HIP(hipMalloc((void**)&g.d_start, sizeof(int)*dg.nblk));
+        SPD_HIP(hipMalloc((void**)&g.d_start, sizeof(int)*kFixupParts*dg.nblk));
         SPD_HIP(hipMemcpy(g.d_tables, g.h_dx.data(), sizeof(double)*3*dg.nblk, hipMemcpyHostToDevice));
         SPD_HIP(hipMemcpy(g.d_tables + 3*dg.nblk, g.h_inv_dx.data(), sizeof(double)*3*dg.nblk, hipMemcpyHostToDevice));
         SPD_HIP(hipMemcpy(g.d_tables + 6*dg.nblk, g.h_bmin.data(), sizeof(double)*3*dg.nblk, hipMemcpyHostToDevice));
         SPD_HIP(hipMemcpy(g.d_nbr, nbr.data(), sizeof(int)*27*dg.nblk, hipMemcpyHostToDevice));
         SPD_HIP(hipMemcpy(g.d_push, push.data(), sizeof(int)*27*dg.nblk, hipMemcpyHostToDevice));
-        const std::vector<int> complete(dg.nblk, dg.nc[2]);
-        SPD_HIP(hipMemcpy(g.d_start, complete.data(), sizeof(int)*dg.nblk, hipMemcpyHostToDevice));
+        const std::vector<int> complete((size_t)kFixupParts*dg.nblk, dg.nc[2]);
+        SPD_HIP(hipMemcpy(g.d_start, complete.data(), sizeof(int)*kFixupParts*dg.nblk, hipMemcpyHostToDevice));
         dg.dx = g.d_tables;
         dg.inv_dx = g.d_tables + 3*dg.nblk;
         dg.bmin = g.d_tables + 6*dg.nblk;

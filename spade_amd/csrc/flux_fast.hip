@@ -380,7 +380,7 @@ struct StageArgs
 template <int KEEP, int PDISS, int VISC, bool INC, bool MAP, int STG = 0>
 __global__ void __launch_bounds__(256, 2)
 k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, FastCfg C,
-            int* __restrict__ start, StageArgs A)
+            int* __restrict__ start, StageArgs A, int zsplit)
 {
     using S = Spec<KEEP, PDISS, VISC>;
     static_assert(!(STG && INC), "a fused stage writes a fresh residual");
@@ -394,26 +394,51 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     // and at faces 0 .. 16 of each axis, from the same map_deriv as the
     // per-cell kernels (so the same bits)
     __shared__ double sMc[MAP ? 3 : 1][18], sMf[MAP ? 3 : 1][17];
+    // mapped y only (x and z identity, so their d x / d xi are 1.0 exactly):
+    // every face normal and Jacobian depends on the row alone, tabulated per
+    // block with the same expressions: sY[0][r] = jac at cell row r - 1,
+    // sY[1][r] = its normal factor 1/(1 jac), sY[2][r] = the y-face normal
+    // factor of a cell row, sY[3][f] = the y-face normal factor at face f
+    __shared__ double sY[MAP ? 4 : 1][18];
     __shared__ double sU[3][3*HP];        // u, v, w of planes k-1, k, k+1 (ring)
     __shared__ double sS[2][4*HP];        // p, T, rho, e of planes k, k+1 (by parity)
     __shared__ double sFx[5][16*17];      // x-face products, faces i = 0..16 of row j
     __shared__ double sFy[5][17*16];      // y-face products, faces j = 0..16 of column i
 
-    const int lb = blockIdx.x;
+    // zsplit 2 or 4: that many workgroups per block, each marching over its
+    // share of the planes [k0, k1) with its own window prologue (fewer, shorter tail rounds when
+    // the block count is not a multiple of the resident workgroups); each
+    // records the first plane it left to the fix-up in its own d_start entry.
+    // The fix-up redoes the block from the first of them to the end, which
+    // overwrites whatever the other half wrote there: the same planes the
+    // one-workgroup launch leaves, so the same bits (overwrite mode only)
+    const int lb = blockIdx.x/zsplit, half = blockIdx.x - lb*zsplit;
+    const int k0 = half*(G.nc[2]/zsplit), k1 = k0 + G.nc[2]/zsplit;
+    int* __restrict__ my_start = start + kFixupParts*lb + half;
     const int tid = threadIdx.x;
+    if (half == 0 && tid < kFixupParts && tid >= zsplit) start[kFixupParts*lb + tid] = G.nc[2];
     const int i = tid & 15, j = tid >> 4;
     const double ix = G.inv_dx[3*lb + 0], iy = G.inv_dx[3*lb + 1], iz = G.inv_dx[3*lb + 2];
+    const bool yonly = MAP && G.map_kind[0] == SPD_MAP_IDENTITY && G.map_kind[2] == SPD_MAP_IDENTITY;
     if constexpr (MAP)
     {
         if (tid < 54)
         {
             const int a = tid/18, c = tid % 18 - 1;
-            sMc[a][c + 1] = map_deriv(G, a, G.bmin[3*lb + a] + (c + 0.5)*G.dx[3*lb + a]);
+            const double m = map_deriv(G, a, G.bmin[3*lb + a] + (c + 0.5)*G.dx[3*lb + a]);
+            sMc[a][c + 1] = m;
+            if (a == 1)
+            {
+                const double jc = jac_at(1.0, m, 1.0);
+                sY[0][c + 1] = jc; sY[1][c + 1] = nrm_at(1.0, jc); sY[2][c + 1] = nrm_at(m, jc);
+            }
         }
         else if (tid < 105)
         {
             const int a = (tid - 54)/17, c = (tid - 54) % 17;
-            sMf[a][c] = map_deriv(G, a, G.bmin[3*lb + a] + ((c + 0.5) - 0.5)*G.dx[3*lb + a]);
+            const double m = map_deriv(G, a, G.bmin[3*lb + a] + ((c + 0.5) - 0.5)*G.dx[3*lb + a]);
+            sMf[a][c] = m;
+            if (a == 1) sY[3][c] = nrm_at(m, jac_at(1.0, m, 1.0));
         }
     }
     const int nz = G.nc[2];
@@ -483,29 +508,29 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     // for the prefetch
     double rin[5];
     #pragma unroll
-    for (int n = 0; n < 5; ++n) rin[n] = INC ? Rown[n*sv] : 0.0;
+    for (int n = 0; n < 5; ++n) rin[n] = INC ? Rown[k0*plane + n*sv] : 0.0;
 
-    // prologue: planes -1, 0, 1 resident; planes 2 and 3 in flight (two raw
-    // register sets used alternately, so a plane's load is never copied and
-    // waiting for it never waits for the later one)
-    const CellV prev = cell(load_own(-1));
-    const CellV cur0 = cell(load_own(0));
-    const CellV nxt0 = cell(load_own(1));
+    // prologue: planes k0-1, k0, k0+1 resident; planes k0+2 and k0+3 in flight
+    // (two raw register sets used alternately, so a plane's load is never
+    // copied and waiting for it never waits for the later one)
+    const CellV prev = cell(load_own(k0 - 1));
+    const CellV cur0 = cell(load_own(k0));
+    const CellV nxt0 = cell(load_own(k0 + 1));
     Raw nA{}, nB{}, hA{}, hB{};
-    if (2 <= nz) nA = load_own(2);
-    if (3 <= nz) nB = load_own(3);
+    if (k0 + 2 <= k1) nA = load_own(k0 + 2);
+    if (k0 + 3 <= k1) nB = load_own(k0 + 3);
     put_vel(ring[0], pown, prev);
     put_vel(ring[1], pown, cur0); put_sca(sS[0], pown, cur0);
     put_vel(ring[2], pown, nxt0); put_sca(sS[1], pown, nxt0);
     if (tid == 0) { sBad[0] = 0; sBad[1] = 0; }
     if (has_halo)
     {
-        put_vel(ring[0], phalo, cell(load_halo(-1)));
-        const CellV h0 = cell(load_halo(0)), h1 = cell(load_halo(1));
+        put_vel(ring[0], phalo, cell(load_halo(k0 - 1)));
+        const CellV h0 = cell(load_halo(k0)), h1 = cell(load_halo(k0 + 1));
         put_vel(ring[1], phalo, h0); put_sca(sS[0], phalo, h0);
         put_vel(ring[2], phalo, h1); put_sca(sS[1], phalo, h1);
-        if (2 <= nz) hA = load_halo(2);
-        if (3 <= nz) hB = load_halo(3);
+        if (k0 + 2 <= k1) hA = load_halo(k0 + 2);
+        if (k0 + 3 <= k1) hB = load_halo(k0 + 3);
     }
     lds_barrier();
 
@@ -518,11 +543,15 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
         grads_z<S>(ring[0], ring[1], pown, prev, cur0, ix, iy, iz, g, gT);
         if constexpr (MAP)
         {
-            // face -1/2: L = plane -1, R = plane 0
-            const double mL = sMc[2][0], mR = sMc[2][1], mf = sMf[2][0];
-            mt[0] = nrm_at(mL, jac_at(m0i, m1j, mL));
-            mt[1] = nrm_at(mR, jac_at(m0i, m1j, mR));
-            mt[2] = nrm_at(mf, jac_at(m0i, m1j, mf));
+            // face k0 - 1/2: L = plane k0 - 1, R = plane k0
+            const double mL = sMc[2][k0], mR = sMc[2][k0 + 1], mf = sMf[2][k0];
+            if (yonly) { mt[0] = mt[1] = mt[2] = sY[1][j + 1]; }
+            else
+            {
+                mt[0] = nrm_at(mL, jac_at(m0i, m1j, mL));
+                mt[1] = nrm_at(mR, jac_at(m0i, m1j, mR));
+                mt[2] = nrm_at(mf, jac_at(m0i, m1j, mf));
+            }
             grads_map<2>(G, g, gT, mf, m0i, m1j);
         }
         face_product<2, S, MAP>(C, prev, cur0, g, gT, iz, mt, zlo, bad);
@@ -535,7 +564,7 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     {
         // the x pushes staged by plane k-1 (visible since its last barrier;
         // the staging buffer is refilled only after this plane's first barrier)
-        if (STG && A.push_ghosts && k > 0) flush_x(G, A.push, A.qout, xs, lb, k - 1);
+        if (STG && A.push_ghosts && k > k0) flush_x(G, A.push, A.qout, xs, lb, k - 1);
         const double* sm = ring[0];
         const double* s0 = ring[1];
         const double* sp = ring[2];
@@ -546,7 +575,7 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
         double zhi[5];
         // mapped coordinates: plane k's d z / d zeta and this cell's Jacobian
         const double m2k = MAP ? sMc[2][k + 1] : 1.0;
-        const double jc = MAP ? jac_at(m0i, m1j, m2k) : 1.0;
+        const double jc = MAP ? (yonly ? sY[0][j + 1] : jac_at(m0i, m1j, m2k)) : 1.0;
         // the three faces of all lanes, then the block-boundary x-faces
         // (wave 2) and y-faces (wave 3); the series pow has no branch, so the
         // three faces form one straight-line block
@@ -559,9 +588,13 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             if constexpr (MAP)
             {
                 const double mL = sMc[0][i], mf = sMf[0][i];
-                mt[0] = nrm_at(mL, jac_at(mL, m1j, m2k));
-                mt[1] = nrm_at(m0i, jc);
-                mt[2] = nrm_at(mf, jac_at(mf, m1j, m2k));
+                if (yonly) { mt[0] = mt[1] = mt[2] = sY[1][j + 1]; }
+                else
+                {
+                    mt[0] = nrm_at(mL, jac_at(mL, m1j, m2k));
+                    mt[1] = nrm_at(m0i, jc);
+                    mt[2] = nrm_at(mf, jac_at(mf, m1j, m2k));
+                }
                 grads_map<0>(G, g, gT, mf, m1j, m2k);
             }
             face_product<0, S, MAP>(C, L, cur, g, gT, ix, mt, P, bad);
@@ -576,9 +609,13 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             if constexpr (MAP)
             {
                 const double mL = sMc[1][j], mf = sMf[1][j];
-                mt[0] = nrm_at(mL, jac_at(m0i, mL, m2k));
-                mt[1] = nrm_at(m1j, jc);
-                mt[2] = nrm_at(mf, jac_at(m0i, mf, m2k));
+                if (yonly) { mt[0] = sY[2][j]; mt[1] = sY[2][j + 1]; mt[2] = sY[3][j]; }
+                else
+                {
+                    mt[0] = nrm_at(mL, jac_at(m0i, mL, m2k));
+                    mt[1] = nrm_at(m1j, jc);
+                    mt[2] = nrm_at(mf, jac_at(m0i, mf, m2k));
+                }
                 grads_map<1>(G, g, gT, mf, m2k, m0i);
             }
             face_product<1, S, MAP>(C, L, cur, g, gT, iy, mt, P, bad);
@@ -591,9 +628,13 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             if constexpr (MAP)
             {
                 const double mR = sMc[2][k + 2], mf = sMf[2][k + 1];
-                mt[0] = nrm_at(m2k, jc);
-                mt[1] = nrm_at(mR, jac_at(m0i, m1j, mR));
-                mt[2] = nrm_at(mf, jac_at(m0i, m1j, mf));
+                if (yonly) { mt[0] = mt[1] = mt[2] = sY[1][j + 1]; }
+                else
+                {
+                    mt[0] = nrm_at(m2k, jc);
+                    mt[1] = nrm_at(mR, jac_at(m0i, m1j, mR));
+                    mt[2] = nrm_at(mf, jac_at(m0i, m1j, mf));
+                }
                 grads_map<2>(G, g, gT, mf, m0i, m1j);
             }
             face_product<2, S, MAP>(C, cur, nxt, g, gT, iz, mt, zhi, bad);
@@ -609,9 +650,13 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             if constexpr (MAP)
             {
                 const double m1 = sMc[1][jj + 1], mL = sMc[0][16], mR = sMc[0][17], mf = sMf[0][16];
-                mt[0] = nrm_at(mL, jac_at(mL, m1, m2k));
-                mt[1] = nrm_at(mR, jac_at(mR, m1, m2k));
-                mt[2] = nrm_at(mf, jac_at(mf, m1, m2k));
+                if (yonly) { mt[0] = mt[1] = mt[2] = sY[1][jj + 1]; }
+                else
+                {
+                    mt[0] = nrm_at(mL, jac_at(mL, m1, m2k));
+                    mt[1] = nrm_at(mR, jac_at(mR, m1, m2k));
+                    mt[2] = nrm_at(mf, jac_at(mf, m1, m2k));
+                }
                 grads_map<0>(G, g, gT, mf, m1, m2k);
             }
             face_product<0, S, MAP>(C, L, Rr, g, gT, ix, mt, P, bad);
@@ -628,9 +673,13 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             if constexpr (MAP)
             {
                 const double m0 = sMc[0][ii + 1], mL = sMc[1][16], mR = sMc[1][17], mf = sMf[1][16];
-                mt[0] = nrm_at(mL, jac_at(m0, mL, m2k));
-                mt[1] = nrm_at(mR, jac_at(m0, mR, m2k));
-                mt[2] = nrm_at(mf, jac_at(m0, mf, m2k));
+                if (yonly) { mt[0] = sY[2][16]; mt[1] = sY[2][17]; mt[2] = sY[3][16]; }
+                else
+                {
+                    mt[0] = nrm_at(mL, jac_at(m0, mL, m2k));
+                    mt[1] = nrm_at(mR, jac_at(m0, mR, m2k));
+                    mt[2] = nrm_at(mf, jac_at(m0, mf, m2k));
+                }
                 grads_map<1>(G, g, gT, mf, m2k, m0);
             }
             face_product<1, S, MAP>(C, L, Rr, g, gT, iy, mt, P, bad);
@@ -644,7 +693,7 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
         {
             // some face of plane k needs the library pow: planes k .. nz-1 are
             // left to the fix-up kernel (nothing of them was written)
-            if (tid == 0) start[lb] = k;
+            if (tid == 0) *my_start = k;
             return false;
         }
         if (tid == 0) sBad[(k + 1) & 1] = 0;
@@ -726,9 +775,9 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             }
         }
 
-        if (k + 1 < nz)
+        if (k + 1 < k1)
         {
-            // plane k+2 (<= nz) enters into the ring slot of plane k-1 and the
+            // plane k+2 (<= k1) enters into the ring slot of plane k-1 and the
             // parity buffer of plane k; its raw registers take plane k+4
             double* s_new = ring[0];
             ring[0] = ring[1]; ring[1] = ring[2]; ring[2] = s_new;
@@ -740,43 +789,78 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             }
             const CellV c2 = cell(nr);
             put_vel(s_new, pown, c2); put_sca(t_new, pown, c2);
-            if (k + 4 <= nz) nr = load_own(k + 4);
+            if (k + 4 <= k1) nr = load_own(k + 4);
             if (has_halo)
             {
                 const CellV h = cell(hr);
                 put_vel(s_new, phalo, h); put_sca(t_new, phalo, h);
-                if (k + 4 <= nz) hr = load_halo(k + 4);
+                if (k + 4 <= k1) hr = load_halo(k + 4);
             }
         }
         lds_barrier();
         return true;
     };
 
-    for (int k = 0; k < nz; k += 2)
+    for (int k = k0; k < k1; k += 2)
     {
         if (!plane_step(k, nA, hA)) return;
-        if (k + 1 < nz && !plane_step(k + 1, nB, hB)) return;
+        if (k + 1 < k1 && !plane_step(k + 1, nB, hB)) return;
     }
-    if (STG && A.push_ghosts) flush_x(G, A.push, A.qout, xs, lb, nz - 1);   // after the last plane's closing barrier
-    if (tid == 0) start[lb] = nz;
+    if (STG && A.push_ghosts) flush_x(G, A.push, A.qout, xs, lb, k1 - 1);   // after the last plane's closing barrier
+    if (tid == 0) *my_start = nz;
+}
+
+// Workgroups per block (1, 2 or 4), each marching over an equal share of the
+// block's planes, chosen by rounds of resident workgroups (two per CU):
+// ceil(zs nblk / slots) rounds, each costing 1/zs of a block's march plus a
+// window prologue of about 0.1 of it.  A share must hold an even number of
+// planes.  SPD_FAST_ZSPLIT=1|2|4 forces the choice.
+int fast_zsplit(const Grid& g)
+{
+    const int nz = g.geom.nc[2];
+    auto ok = [&](int zs) { return nz % (2*zs) == 0; };
+    if (const char* e = getenv("SPD_FAST_ZSPLIT"))
+    {
+        const int zs = e[0] == '4' ? 4 : (e[0] == '2' ? 2 : 1);
+        return ok(zs) ? zs : 1;
+    }
+    static int cus[64] = {};
+    const int dev = g.ctx->device;
+    if (dev < 0 || dev >= 64) return 1;
+    if (!cus[dev]) SPD_HIP(hipDeviceGetAttribute(&cus[dev], hipDeviceAttributeMultiprocessorCount, dev));
+    const long slots = 2L*(cus[dev] > 0 ? cus[dev] : 1), nb = g.geom.nblk;
+    int best = 1;
+    double best_cost = 1e300;
+    for (int zs = 1; zs <= kFixupParts; zs *= 2)
+    {
+        if (!ok(zs)) continue;
+        const double rounds = (double)((zs*nb + slots - 1)/slots);
+        const double cost = rounds*(1.0/zs + (zs > 1 ? 0.1 : 0.0));
+        if (cost < best_cost - 1e-9) { best_cost = cost; best = zs; }
+    }
+    return best;
 }
 
 template <int KEEP, int PDISS, int VISC>
 void launch_fast(const Grid& g, const double* prims, double* rhs, const FastCfg& C, bool increment, hipStream_t s)
 {
+    // increment mode keeps one workgroup per block: the fix-up redoes a block
+    // from its first flagged plane on, which would add the other half's
+    // already-written planes twice
+    const int zs = increment ? 1 : fast_zsplit(g);
     const bool map = !g.geom.identity;
     if (increment && map)
-        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, true, true>), dim3(g.geom.nblk), dim3(256), 0, s,
-                           g.geom, prims, rhs, C, g.d_start, StageArgs{});
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, true, true>), dim3(g.geom.nblk*zs), dim3(256), 0, s,
+                           g.geom, prims, rhs, C, g.d_start, StageArgs{}, zs);
     else if (increment)
-        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, true, false>), dim3(g.geom.nblk), dim3(256), 0, s,
-                           g.geom, prims, rhs, C, g.d_start, StageArgs{});
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, true, false>), dim3(g.geom.nblk*zs), dim3(256), 0, s,
+                           g.geom, prims, rhs, C, g.d_start, StageArgs{}, zs);
     else if (map)
-        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, true>), dim3(g.geom.nblk), dim3(256), 0, s,
-                           g.geom, prims, rhs, C, g.d_start, StageArgs{});
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, true>), dim3(g.geom.nblk*zs), dim3(256), 0, s,
+                           g.geom, prims, rhs, C, g.d_start, StageArgs{}, zs);
     else
-        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, false>), dim3(g.geom.nblk), dim3(256), 0, s,
-                           g.geom, prims, rhs, C, g.d_start, StageArgs{});
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, false>), dim3(g.geom.nblk*zs), dim3(256), 0, s,
+                           g.geom, prims, rhs, C, g.d_start, StageArgs{}, zs);
     SPD_LAUNCH_CHECK();
 }
 
@@ -796,15 +880,16 @@ template <int KEEP, int PDISS, int VISC>
 void launch_fast_stage(const Grid& g, const double* qin, double* rhs, const FastCfg& C, const StageArgs& A,
                        bool read_neighbours, hipStream_t s)
 {
+    const int zs = fast_zsplit(g);
     if (!g.geom.identity)
-        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, true, 1>), dim3(g.geom.nblk), dim3(256), 0, s,
-                           g.geom, qin, rhs, C, g.d_start, A);
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, true, 1>), dim3(g.geom.nblk*zs), dim3(256), 0, s,
+                           g.geom, qin, rhs, C, g.d_start, A, zs);
     else if (read_neighbours)
-        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, false, 2>), dim3(g.geom.nblk), dim3(256), 0, s,
-                           g.geom, qin, rhs, C, g.d_start, A);
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, false, 2>), dim3(g.geom.nblk*zs), dim3(256), 0, s,
+                           g.geom, qin, rhs, C, g.d_start, A, zs);
     else
-        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, false, 1>), dim3(g.geom.nblk), dim3(256), 0, s,
-                           g.geom, qin, rhs, C, g.d_start, A);
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, false, 1>), dim3(g.geom.nblk*zs), dim3(256), 0, s,
+                           g.geom, qin, rhs, C, g.d_start, A, zs);
     SPD_LAUNCH_CHECK();
 }
 

@@ -662,7 +662,7 @@ k_flux_fixup(DevGeom G, const double* __restrict__ prims, double* __restrict__ r
     // that none of this workgroup's blocks needs completing: every thread reads
     // the same flags, so the whole workgroup leaves before staging anything
     bool any = false;
-    for (int lb = blockIdx.x; lb < G.nblk; lb += gridDim.x) any |= start[lb] < nz;
+    for (int lb = blockIdx.x; lb < G.nblk; lb += gridDim.x) any |= fixup_start(start, lb) < nz;
     if (!any) return;
     __shared__ DevGeom sG;                // see k_flux_div_generic
     __shared__ spd_flux_desc sF;
@@ -671,7 +671,7 @@ k_flux_fixup(DevGeom G, const double* __restrict__ prims, double* __restrict__ r
     const int per_plane = G.nc[0]*G.nc[1];
     for (int lb = blockIdx.x; lb < G.nblk; lb += gridDim.x)
     {
-        int s = start[lb];
+        int s = fixup_start(start, lb);
         if (s >= nz) continue;
         if (s < 0) s = 0;
         const int n = per_plane*(nz - s);
@@ -795,7 +795,7 @@ k_flux_fixup_spec(DevGeom G, const double* __restrict__ prims, double* __restric
 {
     const int nz = G.nc[2];
     bool any = false;
-    for (int lb = blockIdx.x; lb < G.nblk; lb += gridDim.x) any |= start[lb] < nz;
+    for (int lb = blockIdx.x; lb < G.nblk; lb += gridDim.x) any |= fixup_start(start, lb) < nz;
     if (!any) return;
     __shared__ DevGeom sG;
     __shared__ spd_flux_desc sF;
@@ -804,7 +804,7 @@ k_flux_fixup_spec(DevGeom G, const double* __restrict__ prims, double* __restric
     const int per_plane = G.nc[0]*G.nc[1];
     for (int lb = blockIdx.x; lb < G.nblk; lb += gridDim.x)
     {
-        int s = start[lb];
+        int s = fixup_start(start, lb);
         if (s >= nz) continue;
         if (s < 0) s = 0;
         const int n = per_plane*(nz - s);

@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256)
 k_exchange_flagged(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a, const int* __restrict__ only)
 {
     const int lb = blockIdx.x;
-    if (only[lb] >= G.nc[2]) return;
+    if (fixup_start(only, lb) >= G.nc[2]) return;
     const int items = exchange_items(G);
     for (int t = threadIdx.x; t < items; t += blockDim.x) exchange_item(G, nbr, a, lb, t);
 }

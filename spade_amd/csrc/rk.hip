@@ -119,7 +119,7 @@ k_stage_fixup(DevGeom G, const double* __restrict__ qin, double* __restrict__ qo
 {
     __shared__ XStage xs;
     const int lb = blockIdx.x;
-    for (int k = start[lb]; k < G.nc[2]; ++k)
+    for (int k = fixup_start(start, lb); k < G.nc[2]; ++k)
     {
         const CellIt ci{(int)(threadIdx.x & 15), (int)(threadIdx.x >> 4), k, lb};
         double qq[5], cons[5];

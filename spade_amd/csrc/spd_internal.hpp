@@ -100,6 +100,18 @@ __device__ inline void lds_barrier()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// The first plane of block lb left to the fix-up kernels by the LDS-staged
+// flux kernel (nz when none): that kernel may split a block's planes between
+// up to kFixupParts workgroups, each recording the first plane it left in its
+// own entry (d_start holds kFixupParts ints per block)
+constexpr int kFixupParts = 4;
+__host__ __device__ inline int fixup_start(const int* start, int lb)
+{
+    int m = start[kFixupParts*lb];
+    for (int e = 1; e < kFixupParts; ++e) m = start[kFixupParts*lb + e] < m ? start[kFixupParts*lb + e] : m;
+    return m;
+}
+
 // ---- kernels' host launchers ----
 void launch_flux_div_generic(const Grid& g, const double* prims, double* rhs, const spd_flux_desc& f,
                              bool increment, hipStream_t s);
