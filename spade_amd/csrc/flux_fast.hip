@@ -573,6 +573,12 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
         const CellV cur = cell_from_lds(sk, s0, pown);
         const CellV nxt = cell_from_lds(sS[(k + 1) & 1], sp, pown);
         double zhi[5];
+        // this cell's own low x- and y-face products (the LDS copies serve
+        // the neighbour cells that share those faces); the residual reads
+        // them from registers where that fits the register budget (plain
+        // flux_div on identity coordinates without pressure dissipation)
+        double px[5], py[5];
+        constexpr bool own_reg = STG == 0 && !MAP && !S::pd;
         // mapped coordinates: plane k's d z / d zeta and this cell's Jacobian
         const double m2k = MAP ? sMc[2][k + 1] : 1.0;
         const double jc = MAP ? (yonly ? sY[0][j + 1] : jac_at(m0i, m1j, m2k)) : 1.0;
@@ -583,7 +589,7 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             // low x-face of (i,j): L = (i-1, j)
             const int pL = pown - 1;
             const CellV L = cell_from_lds(sk, s0, pL);
-            double g[3][3], gT, P[5], mt[3] = {1.0, 1.0, 1.0};
+            double g[3][3], gT, mt[3] = {1.0, 1.0, 1.0};
             grads_x<S>(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
             if constexpr (MAP)
             {
@@ -597,14 +603,14 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
                 }
                 grads_map<0>(G, g, gT, mf, m1j, m2k);
             }
-            face_product<0, S, MAP>(C, L, cur, g, gT, ix, mt, P, bad);
-            for (int n = 0; n < 5; ++n) sFx[n][j*17 + i] = P[n];
+            face_product<0, S, MAP>(C, L, cur, g, gT, ix, mt, px, bad);
+            for (int n = 0; n < 5; ++n) sFx[n][j*17 + i] = px[n];
         }
         {
             // low y-face of (i,j): L = (i, j-1)
             const int pL = pown - HX;
             const CellV L = cell_from_lds(sk, s0, pL);
-            double g[3][3], gT, P[5], mt[3] = {1.0, 1.0, 1.0};
+            double g[3][3], gT, mt[3] = {1.0, 1.0, 1.0};
             grads_y<S>(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
             if constexpr (MAP)
             {
@@ -618,8 +624,8 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
                 }
                 grads_map<1>(G, g, gT, mf, m2k, m0i);
             }
-            face_product<1, S, MAP>(C, L, cur, g, gT, iy, mt, P, bad);
-            for (int n = 0; n < 5; ++n) sFy[n][j*16 + i] = P[n];
+            face_product<1, S, MAP>(C, L, cur, g, gT, iy, mt, py, bad);
+            for (int n = 0; n < 5; ++n) sFy[n][j*16 + i] = py[n];
         }
         {
             // high z-face k+1/2: L = (i,j,k), R = (i,j,k+1)
@@ -687,6 +693,20 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
         }
         #pragma unroll
         for (int n = 0; n < 5; ++n) keep_here(zhi[n]);
+        // a fused stage's earlier residuals of cell (i,j,k), requested before
+        // the barrier so that their latency overlaps the wait for the other waves
+        double rprev[2][5];
+        if constexpr (STG)
+        {
+            const long long co = own_off + k*plane;
+            #pragma unroll
+            for (int t = 0; t < 2; ++t)
+            {
+                if (t + 1 >= A.n || A.c[t] == 0.0) continue;
+                #pragma unroll
+                for (int v = 0; v < 5; ++v) rprev[t][v] = A.r[t][co + v*sv];
+            }
+        }
         if (bad) sBad[k & 1] = 1;
         lds_barrier();
         if (sBad[k & 1])
@@ -727,9 +747,9 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
             }
             else
             {
-                r += sFx[n][j*17 + i];
+                r += own_reg ? px[n] : sFx[n][j*17 + i];
                 r -= sFx[n][j*17 + i + 1];
-                r += sFy[n][j*16 + i];
+                r += own_reg ? py[n] : sFy[n][j*16 + i];
                 r -= sFy[n][(j + 1)*16 + i];
                 r += zlo[n];
                 r -= zhi[n];
@@ -751,7 +771,7 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
                 if (t + 1 >= A.n || A.c[t] == 0.0) continue;
                 double inc[5];
                 #pragma unroll
-                for (int v = 0; v < 5; ++v) inc[v] = A.c[t]*A.r[t][co + v*sv];
+                for (int v = 0; v < 5; ++v) inc[v] = A.c[t]*rprev[t][v];
                 #pragma unroll
                 for (int v = 0; v < 5; ++v) cons[v] += inc[v];
             }
