@@ -6,7 +6,9 @@ leave the power-law series range in the lower half, the upper half or both
 (the fix-up then redoes the block from the first flagged plane, as the
 one-workgroup launch does), the stretched channel (mapped coordinates), and
 the fused SSP-RK3 step; two and four workgroups per block.  SPD_FAST_ZSPLIT=1|2|4
-forces the split.
+forces the split.  The workgroups taken in contiguous runs of blocks per XCD
+(SPD_FAST_XCD=1, the default of the fused stages that read their halo from
+the neighbour blocks) equal the launch order bit for bit.
 """
 import os
 
@@ -125,5 +127,56 @@ def test_split_fused_step_equals_whole(hot, zs):
         it.close()
         return out
     a, b = _with_split(1, run), _with_split(zs, run)
+    assert np.isfinite(a).all()
+    assert _same(a, b)
+
+
+def _with_xcd(x, fn):
+    os.environ["SPD_FAST_XCD"] = str(x)
+    try:
+        return fn()
+    finally:
+        os.environ.pop("SPD_FAST_XCD", None)
+
+
+@pytest.mark.parametrize("zs", [1, 4])
+def test_xcd_runs_flux_div_equals_launch_order(zs):
+    """Workgroups taken in contiguous runs of blocks per XCD (a permutation of
+    the launch order) give the same residual as the launch order, with and
+    without a block split and with a block left to the fix-up; the residual
+    array starts as noise, so a block that no workgroup took would show."""
+    sa = _sa()
+    g, q = _tgv(sa, nb=(4, 3, 5), hot_planes=((7, 9),))
+    init = np.random.default_rng(11).standard_normal(g.host_shape)
+    print("xcd runs flux_div, zsplit", zs, flush=True)
+
+    def run():
+        rhs = sa.GridArray(g).upload(init)
+        sa.flux_div(q, rhs, RawFlux([K2, VP], GAMMA, R), increment=False)
+        return rhs.download()
+    a = _with_split(zs, lambda: _with_xcd(0, run))
+    b = _with_split(zs, lambda: _with_xcd(1, run))
+    assert np.isfinite(a).all()
+    assert _same(a, b)
+
+
+@pytest.mark.parametrize("hot", [(), ((1, 9),)], ids=["plain", "fixup"])
+def test_xcd_runs_fused_step_equals_launch_order(hot):
+    """The fused SSP-RK3 stages (whose neighbour-reading launches take XCD
+    runs by default) equal the same stages in launch order, bit for bit."""
+    sa = _sa()
+    print("xcd runs fused step, hot planes", hot, flush=True)
+
+    def run():
+        g, q = _tgv(sa, nb=(3, 2, 2), hot_planes=hot)
+        gas = sa.IdealGas(GAMMA, R)
+        flux = sa.compose(sa.cent_keep(2, gas), sa.visc_lr(sa.power_law(1.0 / 1600.0, 1.0, 0.76, 0.72), gas))
+        it = sa.Integrator(q, flux, 1e-3)
+        assert it.fused
+        it.advance(3)
+        out = q.download()
+        it.close()
+        return out
+    a, b = _with_xcd(0, run), _with_xcd(1, run)
     assert np.isfinite(a).all()
     assert _same(a, b)
