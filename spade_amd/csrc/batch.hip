@@ -183,8 +183,10 @@ __device__ inline CaseParams uniform_params(const CaseParams& c)
 //        half, each with its own halo rows: more, smaller workgroups per CU);
 //   PFB  planes of conserved state in flight ahead of the one entering the
 //        LDS window (1 or 2);
-//   OCC  waves per SIMD the register budget is sized for (3: 168 VGPRs with a
-//        few spills, 2: 256 VGPRs without).
+//   OCC  waves per SIMD the register budget is sized for (3: 168 VGPRs with
+//        56-212 bytes per lane spilled, 2: up to 256 VGPRs without); a
+//        256-thread workgroup holds 48.5 KB of LDS and a 128-thread one
+//        26.5 KB, so three or six of them fit a CU.
 // Every shape evaluates the same expressions in the same order, and the
 // per-block diagnostics are summed over the same two half-block trees (rows
 // 0-7 and 8-15, 128 leaves each, added on the host), so all shapes give the
@@ -202,16 +204,25 @@ template <int TJ> struct Tile
 template <int LAW, bool HAS_U0, int TJ, int PFB, int OCC>
 __global__ void __launch_bounds__(16*TJ, OCC)
 k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q0, double* __restrict__ Qout,
-        double alpha, double beta, int want_diag)
+        double alpha, double beta, int flags)
 {
     using T = Tile<TJ>;
+    // flags: bit 0 per-block diagnostics, bit 1 work items in XCD runs
+    const bool want_diag = flags & 1;
     constexpr int HP = T::HP, NTILE = 16/TJ;
     __shared__ float  sU[3][3*HP];        // u, v, w (f32) of planes k-1, k, k+1
     __shared__ float  sS[2][4*HP];        // p', T', rho', e' of planes k, k+1 (parity)
     __shared__ double sV[2][3*HP];        // u, v, w (f64) of planes k, k+1 (parity)
-    __shared__ double sFx[5][TJ*17];      // face fluxes, converted to FP64 once by the face's owner
-    __shared__ double sFy[5][(TJ + 1)*16];
-    __shared__ double sRed[2][T::NT];
+    // face fluxes as computed (FP32); the cells widen them to FP64 when they
+    // read them, which is exact, so every shape sums the same doubles
+    __shared__ alignas(16) float sFx[5][TJ*17];
+    __shared__ float sFy[5][(TJ + 1)*16];
+    // the per-block diagnostics tree reuses the x-face buffer after the
+    // k-march (the march's last barrier has passed, and the next work item
+    // writes face fluxes only after its prologue barrier); the window plus
+    // the fluxes then fit three 256-thread or six 128-thread workgroups per CU
+    static_assert(sizeof(double)*2*T::NT <= sizeof(sFx), "diagnostics tree exceeds the x-face buffer");
+    double (*sRed)[T::NT] = reinterpret_cast<double (*)[T::NT]>(&sFx[0][0]);
 
     const int tid = threadIdx.x;
     const int i = tid & 15, j = tid >> 4;                // j: row within the tile
@@ -225,7 +236,11 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
     const int pown = lposb(i, j), phalo = lposb(hi, hj);
     const int nitems = *B.nwork*NTILE;
 
-    for (int w = blockIdx.x; w < nitems; w += gridDim.x)
+    // XCD runs: the workgroups on one XCD take consecutive work items of each
+    // round (neighbour blocks of a case, in case-major order) instead of every
+    // eighth; every item is still taken once
+    const int w0 = (flags & 2) ? xcd_run_order(blockIdx.x, gridDim.x) : blockIdx.x;
+    for (int w = w0; w < nitems; w += gridDim.x)
     {
         // slot and case constants are the same for the whole workgroup: held
         // in scalar registers, not reloaded per lane inside the k-march
@@ -331,7 +346,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 float g[3][3], gT, F[5];
                 gradsf_x<HP>(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
                 face_f32<0, LAW>(P, L, cur, g, gT, F);
-                for (int n = 0; n < 5; ++n) sFx[n][j*17 + i] = (double)F[n];
+                for (int n = 0; n < 5; ++n) sFx[n][j*17 + i] = F[n];
             }
             {
                 const int pL = pown - HXB;
@@ -339,7 +354,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 float g[3][3], gT, F[5];
                 gradsf_y<HP>(sm, s0, sp, pL, pown, L, cur, ix, iy, iz, g, gT);
                 face_f32<1, LAW>(P, L, cur, g, gT, F);
-                for (int n = 0; n < 5; ++n) sFy[n][j*16 + i] = (double)F[n];
+                for (int n = 0; n < 5; ++n) sFy[n][j*16 + i] = F[n];
             }
             if (tid >= T::XB && tid < T::XB + TJ)
             {
@@ -349,7 +364,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 float g[3][3], gT, F[5];
                 gradsf_x<HP>(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
                 face_f32<0, LAW>(P, L, Rr, g, gT, F);
-                for (int n = 0; n < 5; ++n) sFx[n][jj*17 + 16] = (double)F[n];
+                for (int n = 0; n < 5; ++n) sFx[n][jj*17 + 16] = F[n];
             }
             if (tid >= T::YB && tid < T::YB + 16)
             {
@@ -359,7 +374,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 float g[3][3], gT, F[5];
                 gradsf_y<HP>(sm, s0, sp, pL, pR, L, Rr, ix, iy, iz, g, gT);
                 face_f32<1, LAW>(P, L, Rr, g, gT, F);
-                for (int n = 0; n < 5; ++n) sFy[n][TJ*16 + ii] = (double)F[n];
+                for (int n = 0; n < 5; ++n) sFy[n][TJ*16 + ii] = F[n];
             }
             float zhi[5];
             {
@@ -399,8 +414,9 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             for (int n = 0; n < 5; ++n)
             {
                 const double zh = (double)zhi[n];
-                r[n] = __builtin_fma(sFx[n][j*17 + i] - sFx[n][j*17 + i + 1], P.ix,
-                       __builtin_fma(sFy[n][j*16 + i] - sFy[n][(j + 1)*16 + i], P.iy, (zlo[n] - zh)*P.iz));
+                r[n] = __builtin_fma((double)sFx[n][j*17 + i] - (double)sFx[n][j*17 + i + 1], P.ix,
+                       __builtin_fma((double)sFy[n][j*16 + i] - (double)sFy[n][(j + 1)*16 + i], P.iy,
+                                     (zlo[n] - zh)*P.iz));
                 zlo[n] = zh;
             }
             const double d_rho = __builtin_fma(P.rho_ref, DU, r[0]);
@@ -569,10 +585,22 @@ int occupancy(int v)
 
 } // namespace
 
+// Work items in XCD runs (1, the default) or every eighth per XCD (0).  A
+// block reads its halo from the neighbour slots' interiors, so with runs the
+// neighbours of a case share L2 lines: the 300-case TGV / vortex ensemble
+// steps 11.4 -> 12.7-12.9 G cell-updates/s (the tuner then picks
+// three-wave shapes).  SPD_BATCH_XCD=0|1 forces the choice.
+static int batch_xcd_runs()
+{
+    if (const char* e = getenv("SPD_BATCH_XCD")) return e[0] == '1' ? 1 : 0;
+    return 1;
+}
+
 void launch_batch_stage(const BatchDev& B, int law, int variant, const double* Qin, const double* Q0, double* Qout,
                         double alpha, double beta, int want_diag, int grid, hipStream_t s)
 {
     if (variant < 0 || variant >= kBatchShapeVariants) throw ArgError{"batch kernel shape variant out of range"};
+    want_diag = (want_diag ? 1 : 0) | (batch_xcd_runs() ? 2 : 0);
     // stage 1 of the Shu-Osher form (alpha = 0) reads no u0
     const bool u0 = alpha != 0.0;
     if (law == SPD_LAW_POWER)

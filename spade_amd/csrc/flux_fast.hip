@@ -356,19 +356,6 @@ __device__ inline int lpos(int i, int j) { return (j + 1)*HX + (i + 1); }
 
 struct Raw { double p, T, u, v, w; };
 
-// The hardware hands workgroup w of a launch to XCD w mod 8.  With XCD runs
-// (bit 8 of the kernel's split argument) the workgroup numbers are permuted so that each XCD takes one contiguous run
-// of blocks instead (XCD c: the c-th eighth, in order): blocks next to each
-// other in x and y then run on the same XCD at about the same time and share
-// the L2 lines of the halos they read from each other's interiors.  A
-// bijection of [0, n) for every n.
-__device__ inline int xcd_run_order(int w, int n)
-{
-    constexpr int X = 8;
-    const int c = w % X, s = w / X, q = n / X, rem = n % X;
-    return c < rem ? c*(q + 1) + s : rem*(q + 1) + (c - rem)*q + s;
-}
-
 // The RK stage a fused launch applies to each cell once its residual L is
 // known (transform-aware, time-integration/advance.h:56-102):
 //   qout = c2p(p2c(q) + c[0] r[0] + .. + c[n-2] r[n-2] + c[n-1] L)

@@ -213,6 +213,19 @@ __device__ inline double map_deriv(const DevGeom& G, int d, double xi)
     return 1.0;
 }
 
+// The hardware hands workgroup w of a launch to XCD w mod 8.  A launch that
+// takes its work in XCD runs permutes its workgroup numbers with this map so
+// that each XCD takes one contiguous run of work items instead (XCD c: the
+// c-th eighth, in order): blocks next to each other then run on the same XCD
+// at about the same time and share the L2 lines of the halos they read from
+// each other's interiors.  A bijection of [0, n) for every n.
+__device__ inline int xcd_run_order(int w, int n)
+{
+    constexpr int X = 8;
+    const int c = w % X, s = w / X, q = n / X, rem = n % X;
+    return c < rem ? c*(q + 1) + s : rem*(q + 1) + (c - rem)*q + s;
+}
+
 // Index of a stored cell (i, j, k in [-ng, n+ng)) of variable v in block lb.
 __host__ __device__ inline long long cell_index(const DevGeom& g, int lb, int v, int i, int j, int k)
 {

@@ -6,8 +6,9 @@
   stated bar for this form is 1e-5 on kinetic energy);
 * periodic conservation of mass and total energy to round-off;
 * a case's result is bit-for-bit the same alone and in a mix, with or without
-  the hipGraph, after the autotuner ran, and with every stage kernel shape
-  (prefetch depth, register budget) the autotuner chooses between;
+  the hipGraph, after the autotuner ran, with every stage kernel shape
+  (prefetch depth, register budget) the autotuner chooses between, and with
+  the work items taken in XCD runs or every eighth per XCD;
 * admission fails cleanly when a case does not fit the HBM budget.
 """
 import numpy as np
@@ -115,6 +116,33 @@ def test_alone_equals_mix_bitwise():
 
 
 SHAPES = [(r, p, o) for r in (16, 8) for p in (1, 2) for o in (3, 2)]
+
+
+@pytest.mark.parametrize("grid,interleaved", [(300, False), (1024, False), (77, True)])
+def test_xcd_runs_agree_bitwise(grid, interleaved):
+    """Work items in contiguous runs per XCD (the default) or every eighth
+    per XCD: the same states and diagnostics, bit for bit, for persistent
+    grids that are and are not multiples of eight."""
+    import os
+    specs = [sa.CaseSpec("vortex", (48, 32, 16), mach=0.3, reynolds=500.0), _tgv(32, 5),
+             sa.CaseSpec("vortex", (16, 48, 32), mach=0.2, reynolds=900.0), _tgv(16, 5)]
+
+    def run(x):
+        os.environ["SPD_BATCH_XCD"] = str(x)
+        try:
+            b = sa.Batch(512 * MB)
+            b.set_shape(grid, interleaved=interleaved)
+            ids = [b.add(sp) for sp in specs]
+            b.step(5)
+            return [(b.download(c), b.diag(c)) for c in ids]
+        finally:
+            os.environ.pop("SPD_BATCH_XCD", None)
+    print("batch xcd runs, grid", grid, "interleaved", interleaved, flush=True)
+    a, c = run(0), run(1)
+    for (sa_, (ka, ea)), (sc, (kc, ec)) in zip(a, c):
+        assert np.isfinite(sa_).all()
+        assert np.array_equal(sa_, sc)
+        assert np.array_equal(ka, kc) and np.array_equal(ea, ec)
 
 
 @pytest.mark.parametrize("rows,prefetch,occupancy", SHAPES)
