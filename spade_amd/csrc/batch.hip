@@ -291,7 +291,11 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             #pragma unroll
             for (int n = 0; n < 5; ++n) u0[n] = HAS_U0 ? Q0[o + n*4096] : 0.0;
         };
-        load_u0(0);
+        // the three-wave budget (168 VGPRs) requests u0 of plane k just before
+        // the plane's first barrier instead of a plane ahead, so that it is not
+        // live across the face evaluation
+        constexpr bool late_u0 = OCC == 3;
+        if (!late_u0) load_u0(0);
 
         // prologue: planes -1, 0, 1 resident, planes 2 .. 1 + PFB in flight.
         // The own column's conserved state of planes k and k+1 stays in
@@ -403,6 +407,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 ke_acc += 0.5*rcur.r*(uc*uc + vc*vc + wc*wc);
                 en_acc += 0.5*(ox*ox + oy*oy + oz*oz);
             }
+            if (late_u0) load_u0(k);
             lds_barrier();
 
             // FP64 residual sum and stage combination for cell (i,j,k); the
@@ -435,7 +440,7 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
             wprev_u = vk[pown]; wprev_v = vk[HP + pown]; wprev_w = vk[2*HP + pown];
             if (k + 1 < 16)
             {
-                load_u0(k + 1);
+                if (!late_u0) load_u0(k + 1);
                 rcur = rnxt; rnxt = nraw[0]; cur = nxt;
                 const PrimD q2 = to_prim(P, rnxt);
                 nxt = to_f32(P, q2);
