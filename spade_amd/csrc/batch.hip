@@ -184,7 +184,7 @@ __device__ inline CaseParams uniform_params(const CaseParams& c)
 //   PFB  planes of conserved state in flight ahead of the one entering the
 //        LDS window (1 or 2);
 //   OCC  waves per SIMD the register budget is sized for (3: 168 VGPRs with
-//        56-212 bytes per lane spilled, 2: up to 256 VGPRs without); a
+//        56-172 bytes per lane spilled, 2: up to 256 VGPRs without); a
 //        256-thread workgroup holds 48.5 KB of LDS and a 128-thread one
 //        26.5 KB, so three or six of them fit a CU.
 // Every shape evaluates the same expressions in the same order, and the
