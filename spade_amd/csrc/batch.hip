@@ -328,13 +328,13 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
         double wprev_u = qprev.u, wprev_v = qprev.v, wprev_w = qprev.w;   // own column, plane k-1 (f64)
         lds_barrier();
 
-        double zlo[5];                   // low z-face flux of the current plane (FP64 copy)
+        float zlo[5];                    // low z-face flux of the current plane (widened where used)
         {
             float g[3][3], gT, F[5];
             gradsf_z<HP>(sU[slotk(-1)], sU[slotk(0)], pown, prev, cur, ix, iy, iz, g, gT);
             face_f32<2, LAW>(P, prev, cur, g, gT, F);
             #pragma unroll
-            for (int n = 0; n < 5; ++n) zlo[n] = (double)F[n];
+            for (int n = 0; n < 5; ++n) zlo[n] = F[n];
         }
 
         for (int k = 0; k < 16; ++k)
@@ -421,8 +421,8 @@ k_stage(BatchDev B, const double* __restrict__ Qin, const double* __restrict__ Q
                 const double zh = (double)zhi[n];
                 r[n] = __builtin_fma((double)sFx[n][j*17 + i] - (double)sFx[n][j*17 + i + 1], P.ix,
                        __builtin_fma((double)sFy[n][j*16 + i] - (double)sFy[n][(j + 1)*16 + i], P.iy,
-                                     (zlo[n] - zh)*P.iz));
-                zlo[n] = zh;
+                                     ((double)zlo[n] - zh)*P.iz));
+                zlo[n] = zhi[n];
             }
             const double d_rho = __builtin_fma(P.rho_ref, DU, r[0]);
             const double d_E   = __builtin_fma(P.p_ref, DU, __builtin_fma(P.e_ref, d_rho, r[1]));
