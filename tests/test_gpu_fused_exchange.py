@@ -11,6 +11,11 @@ from the same interior -- on every stored cell, for:
   than the ghost layers (a cell feeds the low and the high ghosts at once);
 * one ghost layer (walls, and 16x16 blocks through the LDS-staged flux kernel);
 * 16^3 blocks (the LDS-staged flux kernel), several steps.
+
+The stand-alone exchange itself takes its workgroups in runs of blocks per
+XCD by default; SPD_EXCH_XCD=0 gives the launch order, and both give the same
+array on grids whose workgroup counts are and are not multiples of eight, with
+periodic and non-periodic axes.
 """
 import numpy as np
 import pytest
@@ -92,3 +97,26 @@ def test_fast_kernel_blocks():
     sa = _sa()
     g = sa.CartesianGrid((16, 16, 16), (2, 2, 2), ((0.0, 1.0),) * 3, num_exch=2)
     _check(sa, g, sa.ssprk3, 4)
+
+
+@pytest.mark.parametrize("nb,nc,periodic", [((3, 5, 2), (16, 16, 16), (True, False, True)),
+                                            ((4, 4, 4), (8, 12, 16), (True, True, True)),
+                                            ((5, 1, 3), (6, 6, 6), (False, True, False))],
+                         ids=["walls", "periodic", "thin"])
+def test_exchange_xcd_runs_equal_launch_order(nb, nc, periodic):
+    import os
+    sa = _sa()
+    g = sa.CartesianGrid(nc, nb, ((0.0, 1.0),) * 3, num_exch=2, periodic=periodic)
+    host = np.random.default_rng(17).standard_normal(g.host_shape)
+    print("exchange xcd runs", nb, nc, periodic, flush=True)
+    out = {}
+    for x in ("0", "1"):
+        os.environ["SPD_EXCH_XCD"] = x
+        try:
+            q = sa.GridArray(g).upload(host)
+            sa.create_exchange(q).exchange(q)
+            out[x] = q.download()
+        finally:
+            os.environ.pop("SPD_EXCH_XCD", None)
+    assert not np.array_equal(out["0"], host)
+    assert np.array_equal(out["0"], out["1"])
