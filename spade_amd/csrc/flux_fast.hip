@@ -865,16 +865,19 @@ int fast_zsplit(const Grid& g)
 }
 
 // Workgroup order of a launch: contiguous runs of blocks per XCD (1) or the
-// launch order (0).  Runs pay where a block reads its halo from the
-// neighbour blocks' interiors (fused stages on periodic identity grids): on
-// the 256^3 Taylor-Green step they halve the lines those stages fetch into L2
-// (2.97 -> 1.54 GB per launch) and save 2% of the step; a block that reads
-// its own ghost cells shares nothing with its neighbours, and flux_div times
-// the same either way.  SPD_FAST_XCD=0|1 forces the choice.
-int fast_xcd_runs(bool neighbour_reads)
+// launch order (0).  Runs pay where other workgroups read the same lines:
+// a block that reads its halo from the neighbour blocks' interiors (fused
+// stages on periodic identity grids: on the 256^3 Taylor-Green step they
+// halve the lines those stages fetch into L2, 2.97 -> 1.54 GB per launch, and
+// save 2% of the step), or a block split between workgroups, whose parts
+// re-read the planes next to their shares (the 192x128x96 channel flux_div,
+// 4 parts per block: 1-3.5% faster in alternating runs on one GPU).  A whole block that reads its own ghost
+// cells shares nothing with its neighbours, and flux_div times the same
+// either way.  SPD_FAST_XCD=0|1 forces the choice.
+int fast_xcd_runs(bool shared_reads)
 {
     if (const char* e = getenv("SPD_FAST_XCD")) return e[0] == '1' ? 1 : 0;
-    return neighbour_reads ? 1 : 0;
+    return shared_reads ? 1 : 0;
 }
 
 template <int KEEP, int PDISS, int VISC>
@@ -884,7 +887,7 @@ void launch_fast(const Grid& g, const double* prims, double* rhs, const FastCfg&
     // from its first flagged plane on, which would add the other half's
     // already-written planes twice
     const int zs = increment ? 1 : fast_zsplit(g);
-    const int xr = fast_xcd_runs(false) ? 0x100 : 0;
+    const int xr = fast_xcd_runs(zs > 1) ? 0x100 : 0;
     const bool map = !g.geom.identity;
     if (increment && map)
         hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, true, true>), dim3(g.geom.nblk*zs), dim3(256), 0, s,
@@ -918,7 +921,7 @@ void launch_fast_stage(const Grid& g, const double* qin, double* rhs, const Fast
                        bool read_neighbours, hipStream_t s)
 {
     const int zs = fast_zsplit(g);
-    const int xr = fast_xcd_runs(read_neighbours && g.geom.identity) ? 0x100 : 0;
+    const int xr = fast_xcd_runs((read_neighbours && g.geom.identity) || zs > 1) ? 0x100 : 0;
     if (!g.geom.identity)
         hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, true, 1>), dim3(g.geom.nblk*zs), dim3(256), 0, s,
                            g.geom, qin, rhs, C, g.d_start, A, zs | xr);
