@@ -264,8 +264,8 @@ def channel_measure(sa, reps=20, steps=200):
     out = {"ms": round(ms, 4), "mcell_rhs_per_s": round(cells / (ms * 1e-3) * 1e-6, 1), "cells": cells,
            "kernel": sa.flux_div_kernel(q, flux), "gb_s": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4),
            "note": "192x128x96 stretched channel, cent_keep<2> + pressure_diss_lr + visc_lr power law, FP64"}
-    # the channel case as run: SSP-RK3 at CFL 0.5 with both isothermal walls
-    # filled after every stage, 200 steps in one advance() call after 5 untimed
+    # the channel case as run: SSP-RK3 at CFL 0.5 with both isothermal walls,
+    # 200 steps in one advance() call after 5 untimed
     it = sa.Integrator(q, flux, c["dt"], scheme=sa.ssprk3, walls=(2, 3), t_wall=c["T_wall"])
     it.advance(5)
     g.ctx.sync()
@@ -275,7 +275,9 @@ def channel_measure(sa, reps=20, steps=200):
     out["ssprk3_steps"] = {"steps": steps, "ms_per_step": round(ms_step, 4),
                            "cell_updates_per_s": round(cells / (ms_step * 1e-3), 1), "fused": it.fused,
                            "finite": bool(np.isfinite(q.download()).all()),
-                           "note": "transform-aware SSP-RK3, dt for CFL 0.5, wall fills after every stage"}
+                           "note": "transform-aware SSP-RK3, dt for CFL 0.5; after the first stage of the call the "
+                                   "fused stages read their halo from the neighbour blocks and form the wall ghosts, "
+                                   "the last one writes the ghosts and fills the walls"}
     it.close()
     return out
 

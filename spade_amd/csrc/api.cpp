@@ -517,13 +517,16 @@ int spd_integrator_advance(spd_integrator_t it, int nsteps)
             {
                 // the stages of the branch below, each fused with its flux_div:
                 // q -> alt -> q -> alt, then the two buffers trade places so the
-                // array handle holds the new state (the handle is opaque).  On
-                // periodic identity grids a stage reads its halo from the
-                // neighbour blocks' interiors, so only the last stage of this
-                // call writes ghost cells (the first reads the caller's, as
-                // the separate launches would)
+                // array handle holds the new state (the handle is opaque).  When
+                // x and z are periodic and y is periodic or walled on both
+                // faces, a stage reads its halo from the neighbour blocks'
+                // interiors (forming the wall ghosts itself), so only the last
+                // stage of this call writes ghost cells and fills the walls
+                // (the first reads the caller's, as the separate launches would)
                 if (!it->alt) it->alt = new_array(g);
-                const bool nb = stage_reads_neighbours_ok(g);
+                const bool nb = stage_reads_neighbours_ok(g, it->bc.wall_mask);
+                const int wm = it->bc.wall_mask;
+                const double tw = it->bc.t_wall;
                 double* a = q;
                 double* b = it->alt->a.d;
                 double* r[3] = {it->r[0]->a.d, it->r[1]->a.d, it->r[2]->a.d};
@@ -531,16 +534,16 @@ int spd_integrator_advance(spd_integrator_t it, int nsteps)
                 const bool first = n == 0, last = n + 1 == nsteps;
                 { const double c[1] = {(1.0/1.0)*dt};
                   launch_flux_stage_fast(g, a, b, r[0], it->flux, rp, c, 1, true, gamma, R, g.ctx->stream,
-                                         nb && !first, !nb); }
-                apply_bc(it, b);
+                                         nb && !first, !nb, wm, tw); }
+                if (!nb) apply_bc(it, b);
                 { const double c[2] = {(-3.0/4.0)*dt, (1.0/4.0)*dt};
                   launch_flux_stage_fast(g, b, a, r[1], it->flux, rp, c, 2, true, gamma, R, g.ctx->stream,
-                                         nb, !nb); }
-                apply_bc(it, a);
+                                         nb, !nb, wm, tw); }
+                if (!nb) apply_bc(it, a);
                 { const double c[3] = {(-1.0/12.0)*dt, (-1.0/12.0)*dt, (2.0/3.0)*dt};
                   launch_flux_stage_fast(g, a, b, r[2], it->flux, rp, c, 3, false, gamma, R, g.ctx->stream,
-                                         nb, !nb || last); }
-                apply_bc(it, b);
+                                         nb, !nb || last, wm, tw); }
+                if (!nb || last) apply_bc(it, b);
                 std::swap(it->q->a.base, it->alt->a.base);
                 std::swap(it->q->a.d, it->alt->a.d);
                 q = it->q->a.d;

@@ -372,6 +372,8 @@ struct StageArgs
     const int* push;             // push table (d_push)
     int push_ghosts;             // write the ghosts this stage's result feeds
     const int* nbr;              // source table (d_nbr), read halos from neighbour interiors (STG == 2)
+    double t_wall;               // isothermal wall temperature (wall_y)
+    int wall_y;                  // STG == 2: y faces without a source block are isothermal walls
 };
 
 // STG: 0 flux_div only; 1 fused stage, halo from this block's ghost cells;
@@ -384,7 +386,6 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
 {
     using S = Spec<KEEP, PDISS, VISC>;
     static_assert(!(STG && INC), "a fused stage writes a fresh residual");
-    static_assert(!(STG == 2 && MAP), "neighbour-read stages are built for identity coordinates");
     // x-ghost pushes of a fused stage: the plane's cells within ng of an x
     // face, staged for whole-row writes (flushed after the next barrier)
     __shared__ double sXs[STG ? sizeof(XStage)/sizeof(double) : 1];
@@ -475,8 +476,21 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     // exchange would have copied into the ghosts)
     const int hex = hi < 0 ? -1 : (hi > 15 ? 1 : 0), hey = hj < 0 ? -1 : (hj > 15 ? 1 : 0);
     const int* __restrict__ nbl = STG == 2 ? A.nbr + 27*lb : nullptr;
+    // a y-halo cell across an isothermal wall (no source block in y): what
+    // the wall fill writes there, i.e. the cell of the mirrored row (0 for the
+    // low wall, nc1 - 1 for the high one) in the same column and plane,
+    // itself found through the source table without the y step, with
+    // T -> 2 T_wall - T and the velocity negated (boundary_fill.h)
+    // (built into the mapped variant only, where the walled channel runs)
+    const bool hwall = STG == 2 && MAP && has_halo && A.wall_y && hey != 0 && nbl[(hex + 1) + 3*(hey + 1) + 9] < 0;
+    const int hey_s = hwall ? 0 : hey;
+    const int hj_s = hwall ? (hey < 0 ? -1 - hj : 2*G.nc[1] - (hj + 1)) : hj - 16*hey;
+    // (a mirrored cell of the block's own columns lies in the block itself:
+    // the source table has no entry for the block's own region)
+    const int src_n = (STG == 2 && has_halo)
+        ? ((hex == 0 && hey_s == 0) ? lb : nbl[(hex + 1) + 3*(hey_s + 1) + 9]) : 0;
     const double* __restrict__ Bhalo_n = (STG == 2 && has_halo)
-        ? prims + (long long)nbl[(hex + 1) + 3*(hey + 1) + 9]*G.sb + stored_offset(G, hi - 16*hex, hj - 16*hey, 0)
+        ? prims + (long long)src_n*G.sb + stored_offset(G, hi - 16*hex, hj_s, 0)
         : Bhalo;
     auto load_ptr = [&](const double* e) -> Raw { return Raw{e[0], e[sv], e[2*sv], e[3*sv], e[4*sv]}; };
     auto load_own = [&](int ck) -> Raw
@@ -489,10 +503,16 @@ k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rh
     auto load_halo = [&](int ck) -> Raw
     {
         if (STG != 2) return load_ptr(Bhalo + ck*hplane);
-        if (ck >= 0 && ck < nz) return load_ptr(Bhalo_n + ck*plane);
-        const int ez = ck < 0 ? -1 : 1;
-        const long long S = nbl[(hex + 1) + 3*(hey + 1) + 9*(ez + 1)];
-        return load_ptr(prims + S*G.sb + stored_offset(G, hi - 16*hex, hj - 16*hey, ck - ez*nz));
+        Raw r;
+        if (ck >= 0 && ck < nz) r = load_ptr(Bhalo_n + ck*plane);
+        else
+        {
+            const int ez = ck < 0 ? -1 : 1;
+            const long long S = nbl[(hex + 1) + 3*(hey_s + 1) + 9*(ez + 1)];
+            r = load_ptr(prims + S*G.sb + stored_offset(G, hi - 16*hex, hj_s, ck - ez*nz));
+        }
+        if (MAP && hwall) r = Raw{r.p, 2.0*A.t_wall - r.T, -r.u, -r.v, -r.w};
+        return r;
     };
     const long long own_off = blk + stored_offset(G, i, j, 0);   // core column (i, j) at k = 0
     double* __restrict__ Rown = rhs + own_off;
@@ -921,8 +941,11 @@ void launch_fast_stage(const Grid& g, const double* qin, double* rhs, const Fast
                        bool read_neighbours, hipStream_t s)
 {
     const int zs = fast_zsplit(g);
-    const int xr = fast_xcd_runs((read_neighbours && g.geom.identity) || zs > 1) ? 0x100 : 0;
-    if (!g.geom.identity)
+    const int xr = fast_xcd_runs(read_neighbours || zs > 1) ? 0x100 : 0;
+    if (!g.geom.identity && read_neighbours)
+        hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, true, 2>), dim3(g.geom.nblk*zs), dim3(256), 0, s,
+                           g.geom, qin, rhs, C, g.d_start, A, zs | xr);
+    else if (!g.geom.identity)
         hipLaunchKernelGGL((k_flux_fast<KEEP, PDISS, VISC, false, true, 1>), dim3(g.geom.nblk*zs), dim3(256), 0, s,
                            g.geom, qin, rhs, C, g.d_start, A, zs | xr);
     else if (read_neighbours)
@@ -1038,9 +1061,17 @@ static bool stage_geometry(const DevGeom& G)
     return G.nc[0] == 16 && G.nc[1] == 16 && G.ng >= 1 && G.ng <= 2 && G.nc[2] >= 2*G.ng;
 }
 
-bool stage_reads_neighbours_ok(const Grid& g)
+// A stage can read its halo from the neighbour blocks' interiors when every
+// ghost region has a source block (each axis periodic) or, on diagonal
+// (mapped) coordinates, along y only is an isothermal wall the caller fills on
+// both faces (wall_mask bits 2 and 3), whose ghosts the kernel forms from the
+// mirrored cells (the walled channel; the identity-coordinate kernel is built
+// without the wall step)
+bool stage_reads_neighbours_ok(const Grid& g, int wall_mask)
 {
-    return g.geom.identity && g.desc.periodic[0] && g.desc.periodic[1] && g.desc.periodic[2];
+    const bool xz = g.desc.periodic[0] && g.desc.periodic[2];
+    if (g.geom.identity) return xz && g.desc.periodic[1];
+    return xz && (g.desc.periodic[1] || ((wall_mask >> 2) & 3) == 3);
 }
 
 bool flux_stage_fast_eligible(const Grid& g, const spd_flux_desc& f)
@@ -1053,15 +1084,16 @@ bool flux_stage_fast_eligible(const Grid& g, const spd_flux_desc& f)
 
 bool launch_flux_stage_fast(const Grid& g, const double* qin, double* qout, double* rhs, const spd_flux_desc& f,
                             const double* const* rprev, const double* coef, int nr, bool store_rhs,
-                            double gamma, double R, hipStream_t s, bool read_neighbours, bool push)
+                            double gamma, double R, hipStream_t s, bool read_neighbours, bool push,
+                            int wall_mask, double t_wall)
 {
     int keep, pd, vs;
     FastCfg C;
     if (!stage_geometry(g.geom) || !select_fast(g.geom, f, keep, pd, vs, C)) return false;
     if (nr < 1 || nr > 3) throw ArgError{"a fused stage combines 1..3 residuals"};
     if (qin == qout) throw ArgError{"a fused stage needs distinct input and output arrays"};
-    if (read_neighbours && !stage_reads_neighbours_ok(g))
-        throw ArgError{"neighbour-read stages need identity coordinates and every axis periodic"};
+    if (read_neighbours && !stage_reads_neighbours_ok(g, wall_mask))
+        throw ArgError{"neighbour-read stages need every axis periodic, or on mapped coordinates y walled on both faces"};
     StageArgs A{};
     A.qout = qout;
     for (int t = 0; t + 1 < nr; ++t) A.r[t] = rprev[t];
@@ -1072,6 +1104,8 @@ bool launch_flux_stage_fast(const Grid& g, const double* qin, double* qout, doub
     A.push = g.d_push;
     A.push_ghosts = push ? 1 : 0;
     A.nbr = g.d_nbr;
+    A.wall_y = (read_neighbours && !g.desc.periodic[1]) ? 1 : 0;
+    A.t_wall = t_wall;
     bool ok = false;
     switch (keep*3 + pd)
     {
@@ -1091,7 +1125,14 @@ bool launch_flux_stage_fast(const Grid& g, const double* qin, double* qout, doub
         // the per-cell fix-up reads its block's ghost cells: after a
         // neighbour-read stage they may be stale, so the flagged blocks pull
         // theirs first (what an exchange copies; the others exit at once)
-        if (read_neighbours) launch_exchange_flagged(g, const_cast<double*>(qin), s);
+        if (read_neighbours)
+        {
+            launch_exchange_flagged(g, const_cast<double*>(qin), s);
+            // and the wall ghosts from the cells just pulled (the other
+            // blocks' wall ghosts of qin are not read by this stage)
+            if (A.wall_y)
+                for (int face = 2; face < 4; ++face) launch_wall_fill(g, const_cast<double*>(qin), face, t_wall, s);
+        }
         launch_flux_div_fixup(g, qin, rhs, f, false, s);
         const double* r[3] = {rprev && nr > 1 ? rprev[0] : nullptr, rprev && nr > 2 ? rprev[1] : nullptr, nullptr};
         r[nr - 1] = rhs;

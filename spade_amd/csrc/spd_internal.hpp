@@ -126,7 +126,8 @@ bool flux_div_fast_eligible(const Grid& g, const spd_flux_desc& f);
 // planes left to the fix-up, always).  push: the ghosts the result feeds are
 // written into qout (else qout's ghost cells are left alone).
 // read_neighbours: L(qin) takes its halo from the neighbour blocks' interiors
-// instead of qin's ghost cells (identity coordinates, every axis periodic:
+// instead of qin's ghost cells (x and z periodic, y periodic or isothermal
+// walls on both faces, whose ghosts the kernel forms at t_wall:
 // stage_reads_neighbours_ok), so qin's ghosts need not be current.
 // Bit-identical to run_flux_div into rhs followed by launch_rk_transform_stage
 // on qin (whose ghosts hold what an exchange copies).  Needs a fast-kernel
@@ -134,9 +135,10 @@ bool flux_div_fast_eligible(const Grid& g, const spd_flux_desc& f);
 // eligible (nothing launched).  qin and qout must be distinct arrays.
 bool launch_flux_stage_fast(const Grid& g, const double* qin, double* qout, double* rhs, const spd_flux_desc& f,
                             const double* const* rprev, const double* coef, int nr, bool store_rhs,
-                            double gamma, double R, hipStream_t s, bool read_neighbours, bool push);
+                            double gamma, double R, hipStream_t s, bool read_neighbours, bool push,
+                            int wall_mask = 0, double t_wall = 0.0);
 bool flux_stage_fast_eligible(const Grid& g, const spd_flux_desc& f);
-bool stage_reads_neighbours_ok(const Grid& g);
+bool stage_reads_neighbours_ok(const Grid& g, int wall_mask);
 // non-zero when the composition has a compile-time specialised per-cell kernel
 int flux_div_spec_code(const spd_flux_desc& f);
 // the composition contains a hybrid_scheme_t term

@@ -122,7 +122,14 @@ def test_fused_with_pow_fixup_equals_separate():
     assert np.array_equal(a, b)
 
 
-def test_fused_channel_equals_separate():
+@pytest.mark.parametrize("steps,hot", [((2, 1), False), ((1, 3), False), ((1, 2), True)],
+                         ids=["2x1", "1x3", "1x2_fixup"])
+def test_fused_channel_equals_separate(steps, hot):
+    """The walled stretched channel: after the first stage of a call, the
+    stages read their halo from the neighbour blocks and form the wall ghosts
+    themselves; hot: blocks next to both walls whose face temperatures leave
+    the power-law series range (their planes completed by the fix-up kernels,
+    after the flagged blocks' ghosts are pulled and their walls filled)."""
     from oracle import pyoracle as po
     from spade_amd import studies
     sa = _sa()
@@ -130,6 +137,11 @@ def test_fused_channel_equals_separate():
     og = po.make_grid(c["nb"], c["nc"], c["ng"], c["lo"], c["hi"], periodic=(1, 0, 1),
                       maps=(None, (po.MAP_TANH,) + tuple(c["tanh_y"]), None))
     start = c["host"].copy()
+    if hot:
+        nbx, nby = c["nb"][0], c["nb"][1]
+        for bx in range(3):
+            start[bx + nbx * 0, 2:-2, 2:6, 2:-2, 1] *= 1.15                  # bottom row of blocks, near the wall
+            start[bx + nbx * (nby - 1), 2:-2, -6:-2, 2:-2, 1] *= 1.15       # top row
     po.exchange(og, start)
     po.boundary_fill_wall(og, start, 2, c["T_wall"])
     po.boundary_fill_wall(og, start, 3, c["T_wall"])
@@ -143,8 +155,11 @@ def test_fused_channel_equals_separate():
                           sa.visc_lr(sa.power_law(c["mu"], 1.0, c["power"], c["prandtl"]), gas))
         q = sa.GridArray(g).upload(start)
         return q, sa.Integrator(q, flux, c["dt"], scheme=sa.ssprk3, walls=(2, 3), t_wall=c["T_wall"])
-    print("fused vs separate: channel 192x128x96, 2 steps", flush=True)
-    assert np.array_equal(_run(sa, make, (2, 1), True), _run(sa, make, (2, 1), False))
+    print("fused vs separate: channel 192x128x96, steps", steps, "hot", hot, flush=True)
+    a = _run(sa, make, steps, True)
+    b = _run(sa, make, steps, False)
+    assert np.isfinite(a).all()
+    assert np.array_equal(a, b)
 
 
 def test_not_fused_without_ghost_coverage():
