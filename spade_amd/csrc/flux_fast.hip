@@ -377,8 +377,9 @@ struct StageArgs
 };
 
 // STG: 0 flux_div only; 1 fused stage, halo from this block's ghost cells;
-// 2 fused stage, halo from the neighbour blocks' interiors (identity
-// coordinates; every ghost region has a source block)
+// 2 fused stage, halo from the neighbour blocks' interiors (every ghost
+// region has a source block, or, in the mapped kernel, the y regions without
+// one are isothermal walls whose ghosts are formed from the mirrored cells)
 template <int KEEP, int PDISS, int VISC, bool INC, bool MAP, int STG = 0>
 __global__ void __launch_bounds__(256, 2)
 k_flux_fast(DevGeom G, const double* __restrict__ prims, double* __restrict__ rhs, FastCfg C,

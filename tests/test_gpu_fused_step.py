@@ -3,18 +3,20 @@ separate flux_div + stage launches bit for bit, ghost cells included
 (time-integration/advance.h:56-102 applied to rhs = flux_div).
 
 The fused stage writes its result into a second buffer (the integrator's)
-and the two buffers trade places after each step.  On periodic identity
-grids a stage reads its halo from the neighbour blocks' interiors and only
-the last stage of an advance() call writes the ghost cells; elsewhere every
-stage pushes the ghosts its result feeds.  The cases cover one and several
+and the two buffers trade places after each step.  On periodic grids, and
+on the mapped channel (periodic in x and z, isothermal walls in y), a stage
+after the first of an advance() call reads its halo from the neighbour
+blocks' interiors (forming the wall ghosts itself) and only the last stage
+writes the ghost cells and fills the walls; elsewhere every stage pushes the
+ghosts its result feeds.  The cases cover one and several
 steps per call (the state alternating between the buffers), a periodic
 Taylor-Green grid, blocks of 16 x 16 x 8 and block rows that are their own
 neighbours, ghost cells the caller left unexchanged (read by the first
 stage, as the separate launches do), a state whose face temperatures leave
 the power-law series range in some blocks (those planes are completed by the
 fix-up kernels of the fused launch, after their blocks' ghosts are pulled),
-and the stretched channel with isothermal walls (the wall fill runs on the
-stage's output).
+and the stretched channel with isothermal walls (one and several steps per
+call, and blocks at both walls left to the fix-up).
 SPD_DISABLE_FUSED=1 selects the separate launches.
 """
 import os
