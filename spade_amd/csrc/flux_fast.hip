@@ -1126,14 +1126,9 @@ bool launch_flux_stage_fast(const Grid& g, const double* qin, double* qout, doub
         // the per-cell fix-up reads its block's ghost cells: after a
         // neighbour-read stage they may be stale, so the flagged blocks pull
         // theirs first (what an exchange copies; the others exit at once)
-        if (read_neighbours)
-        {
-            launch_exchange_flagged(g, const_cast<double*>(qin), s);
-            // and the wall ghosts from the cells just pulled (the other
-            // blocks' wall ghosts of qin are not read by this stage)
-            if (A.wall_y)
-                for (int face = 2; face < 4; ++face) launch_wall_fill(g, const_cast<double*>(qin), face, t_wall, s);
-        }
+        // (and, on the walled channel, fill their wall ghost slabs from the
+        // cells just pulled)
+        if (read_neighbours) launch_exchange_flagged(g, const_cast<double*>(qin), s, A.wall_y, t_wall);
         launch_flux_div_fixup(g, qin, rhs, f, false, s);
         const double* r[3] = {rprev && nr > 1 ? rprev[0] : nullptr, rprev && nr > 2 ? rprev[1] : nullptr, nullptr};
         r[nr - 1] = rhs;

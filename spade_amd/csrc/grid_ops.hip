@@ -123,33 +123,15 @@ k_exchange(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a, int x
     exchange_item(G, nbr, a, lb, part*blockDim.x + threadIdx.x);
 }
 
-// The exchange of the blocks the LDS-staged flux kernel left to the fix-up
-// (only[lb] < nz): one workgroup per block, the others exit at once
-__global__ void __launch_bounds__(256)
-k_exchange_flagged(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a, const int* __restrict__ only)
-{
-    const int lb = blockIdx.x;
-    if (fixup_start(only, lb) >= G.nc[2]) return;
-    const int items = exchange_items(G);
-    for (int t = threadIdx.x; t < items; t += blockDim.x) exchange_item(G, nbr, a, lb, t);
-}
-
 // boundary_fill with the isothermal no-slip mirror (grid/boundary_fill.h:
 // 104-130): ghost (normal index < 0 or >= n, tangential over the full stored
-// range) = (p, 2 Tw - T, -u, -v, -w) of the mirror cell.
-__global__ void __launch_bounds__(256)
-k_wall_fill(DevGeom G, double* __restrict__ a, int idir, int pm, double t_wall)
+// range) = (p, 2 Tw - T, -u, -v, -w) of the mirror cell.  Cell c of the ghost
+// slab of face (idir, pm) of block lb.
+__device__ inline void wall_cell(const DevGeom& G, double* __restrict__ a, int lb, int idir, int pm, int c,
+                                 double t_wall)
 {
-    // blockIdx.y enumerates the blocks on the face: (b[d1], b[d2]), d1 < d2 the other axes
-    const int d1 = idir == 0 ? 1 : 0, d2 = idir == 2 ? 1 : 2;
-    int b[3];
-    b[idir] = pm == 0 ? 0 : G.nb[idir] - 1;
-    b[d1] = blockIdx.y % G.nb[d1];
-    b[d2] = blockIdx.y / G.nb[d1];
-    const int lb = b[0] + G.nb[0]*(b[1] + G.nb[1]*b[2]);
     int ext[3] = {G.N[0], G.N[1], G.N[2]};
     ext[idir] = G.ng;
-    const int c = blockIdx.x*blockDim.x + threadIdx.x;
     if (c >= ext[0]*ext[1]*ext[2]) return;
     int idx[3] = {c % ext[0] - G.ng, (c / ext[0]) % ext[1] - G.ng, c / (ext[0]*ext[1]) - G.ng};
     if (pm == 0) idx[idir] = -G.ng + (idx[idir] + G.ng);                 // [-ng, 0)
@@ -166,6 +148,46 @@ k_wall_fill(DevGeom G, double* __restrict__ a, int idir, int pm, double t_wall)
     a[cell_index(G, lb, 2, idx[0], idx[1], idx[2])] = -u;
     a[cell_index(G, lb, 3, idx[0], idx[1], idx[2])] = -v;
     a[cell_index(G, lb, 4, idx[0], idx[1], idx[2])] = -w;
+}
+
+// The exchange of the blocks the LDS-staged flux kernel left to the fix-up
+// (only[lb] < nz): one workgroup per block, the others exit at once.  With
+// wall_y (isothermal walls on both y faces), a flagged block at a wall then
+// fills that wall's ghost slab, as the wall fill after an exchange would: the
+// slab mirrors rows of the block, x- and z-ghost cells included, which this
+// workgroup has just written (visible to it after the barrier)
+__global__ void __launch_bounds__(256)
+k_exchange_flagged(DevGeom G, const int* __restrict__ nbr, double* __restrict__ a, const int* __restrict__ only,
+                   int wall_y, double t_wall)
+{
+    const int lb = blockIdx.x;
+    if (fixup_start(only, lb) >= G.nc[2]) return;
+    const int items = exchange_items(G);
+    for (int t = threadIdx.x; t < items; t += blockDim.x) exchange_item(G, nbr, a, lb, t);
+    if (!wall_y) return;
+    const int by = (lb / G.nb[0]) % G.nb[1];
+    const bool low = by == 0, high = by == G.nb[1] - 1;
+    if (!low && !high) return;                       // the same for the whole workgroup
+    __syncthreads();
+    const int n = G.N[0]*G.ng*G.N[2];
+    for (int pm = 0; pm < 2; ++pm)
+    {
+        if (pm == 0 ? !low : !high) continue;
+        for (int c = threadIdx.x; c < n; c += blockDim.x) wall_cell(G, a, lb, 1, pm, c, t_wall);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+k_wall_fill(DevGeom G, double* __restrict__ a, int idir, int pm, double t_wall)
+{
+    // blockIdx.y enumerates the blocks on the face: (b[d1], b[d2]), d1 < d2 the other axes
+    const int d1 = idir == 0 ? 1 : 0, d2 = idir == 2 ? 1 : 2;
+    int b[3];
+    b[idir] = pm == 0 ? 0 : G.nb[idir] - 1;
+    b[d1] = blockIdx.y % G.nb[d1];
+    b[d2] = blockIdx.y / G.nb[d1];
+    const int lb = b[0] + G.nb[0]*(b[1] + G.nb[1]*b[2]);
+    wall_cell(G, a, lb, idir, pm, blockIdx.x*blockDim.x + threadIdx.x, t_wall);
 }
 
 // Layout conversion on the device for nb blocks starting at block lb0:
@@ -416,10 +438,10 @@ void launch_exchange(const Grid& g, double* a, hipStream_t s)
     SPD_LAUNCH_CHECK();
 }
 
-void launch_exchange_flagged(const Grid& g, double* a, hipStream_t s)
+void launch_exchange_flagged(const Grid& g, double* a, hipStream_t s, int wall_y, double t_wall)
 {
     hipLaunchKernelGGL(k_exchange_flagged, dim3(g.geom.nblk), dim3(256), 0, s, g.geom, g.d_nbr, a,
-                       (const int*)g.d_start);
+                       (const int*)g.d_start, wall_y, t_wall);
     SPD_LAUNCH_CHECK();
 }
 

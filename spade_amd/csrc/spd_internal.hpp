@@ -144,8 +144,9 @@ int flux_div_spec_code(const spd_flux_desc& f);
 // the composition contains a hybrid_scheme_t term
 bool flux_has_hybrid(const spd_flux_desc& f);
 void launch_exchange(const Grid& g, double* a, hipStream_t s);
-// the exchange of the blocks with d_start[lb] < nz only (the others exit at once)
-void launch_exchange_flagged(const Grid& g, double* a, hipStream_t s);
+// the exchange of the blocks with d_start[lb] < nz only (the others exit at
+// once); wall_y: those at a y wall then fill its isothermal ghost slab at t_wall
+void launch_exchange_flagged(const Grid& g, double* a, hipStream_t s, int wall_y = 0, double t_wall = 0.0);
 void launch_wall_fill(const Grid& g, double* a, int face, double t_wall, hipStream_t s);
 // boundary_fill over the faces of face_mask (bit 2*dir+pm), one launch per face in face order
 void launch_fill_extrap(const Grid& g, double* a, int face_mask, int order, hipStream_t s);
