@@ -1127,12 +1127,21 @@ bool launch_flux_stage_fast(const Grid& g, const double* qin, double* qout, doub
         // neighbour-read stage they may be stale, so the flagged blocks pull
         // theirs first (what an exchange copies; the others exit at once)
         // (and, on the walled channel, fill their wall ghost slabs from the
-        // cells just pulled)
-        if (read_neighbours) launch_exchange_flagged(g, const_cast<double*>(qin), s, A.wall_y, t_wall);
-        launch_flux_div_fixup(g, qin, rhs, f, false, s);
+        // cells just pulled).  One launch does all of it for the compositions
+        // with a specialised per-cell kernel (every one the fast kernel
+        // takes); SPD_SPLIT_FIXUP=1 keeps the three launches
         const double* r[3] = {rprev && nr > 1 ? rprev[0] : nullptr, rprev && nr > 2 ? rprev[1] : nullptr, nullptr};
         r[nr - 1] = rhs;
-        launch_stage_fixup(g, qin, qout, r, coef, nr, gamma, R, s);
+        bool merged = false;
+        if (const char* e = getenv("SPD_SPLIT_FIXUP"); !(e && e[0] == '1'))
+            merged = launch_stage_fixup_spec(g, const_cast<double*>(qin), qout, rhs, f, r, coef, nr, gamma, R,
+                                             read_neighbours, A.wall_y, t_wall, s);
+        if (!merged)
+        {
+            if (read_neighbours) launch_exchange_flagged(g, const_cast<double*>(qin), s, A.wall_y, t_wall);
+            launch_flux_div_fixup(g, qin, rhs, f, false, s);
+            launch_stage_fixup(g, qin, qout, r, coef, nr, gamma, R, s);
+        }
     }
     return true;
 }

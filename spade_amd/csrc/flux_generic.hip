@@ -12,6 +12,7 @@
 // kernel in flux_fast.hip for identity coordinates, and elsewhere (mapped
 // coordinates) a compile-time specialised form of this kernel, k_flux_div_spec.
 #include "spd_internal.hpp"
+#include "spd_stage.hpp"
 #include <cmath>
 #include <algorithm>
 
@@ -816,6 +817,79 @@ k_flux_fixup_spec(DevGeom G, const double* __restrict__ prims, double* __restric
     }
 }
 
+// The fix-up of a fused stage in one launch, one workgroup per block (the
+// blocks the LDS-staged kernel completed exit at once).  For a block left
+// from plane s on: with pull, its ghost cells as the exchange writes them,
+// then (wall_y) its isothermal wall slabs from those cells; the residual of
+// planes s.. by the composition's per-cell code (k_flux_fixup_spec); then
+// the stage of those planes, qin -> qout, with the ghosts they feed
+// (k_stage_fixup).  Each phase reads only what this workgroup wrote before
+// the barrier between them, so the result equals the three launches'.
+template <int T0, int T1, int T2>
+__global__ void __launch_bounds__(256) __attribute__((flatten))
+k_stage_fixup_spec(DevGeom G, double* __restrict__ qin, double* __restrict__ qout, double* __restrict__ rhs,
+                   spd_flux_desc F, Coefs C, double gamma, double R, const int* __restrict__ push,
+                   const int* __restrict__ nbr, const int* __restrict__ start, int pull, int wall_y, double t_wall)
+{
+    const int lb = blockIdx.x, nz = G.nc[2];
+    int s = fixup_start(start, lb);
+    if (s >= nz) return;
+    if (s < 0) s = 0;
+    __shared__ DevGeom sG;
+    __shared__ spd_flux_desc sF;
+    __shared__ XStage xs;
+    if (threadIdx.x == 0) { sG = G; sF = F; }
+    if (pull)
+    {
+        const int items = exchange_items(G);
+        for (int t = threadIdx.x; t < items; t += blockDim.x) exchange_item(G, nbr, qin, lb, t);
+        const int by = (lb / G.nb[0]) % G.nb[1];
+        const bool low = by == 0, high = by == G.nb[1] - 1;
+        if (wall_y && (low || high))
+        {
+            __syncthreads();
+            const int n = G.N[0]*G.ng*G.N[2];
+            for (int pm = 0; pm < 2; ++pm)
+            {
+                if (pm == 0 ? !low : !high) continue;
+                for (int c = threadIdx.x; c < n; c += blockDim.x) wall_cell(G, qin, lb, 1, pm, c, t_wall);
+            }
+        }
+    }
+    __syncthreads();
+    const int per_plane = G.nc[0]*G.nc[1];
+    const int n = per_plane*(nz - s);
+    for (int c = threadIdx.x; c < n; c += blockDim.x)
+    {
+        const int i = c % G.nc[0], j = (c / G.nc[0]) % G.nc[1], k = s + c/per_plane;
+        spec_cell<T0, T1, T2>(sG, qin, rhs, sF, 0, i, j, k, lb);
+    }
+    __syncthreads();
+    for (int k = s; k < nz; ++k)
+    {
+        const CellIt ci{(int)(threadIdx.x & 15), (int)(threadIdx.x >> 4), k, lb};
+        double qq[5], cons[5];
+        long long idx[5];
+        for (int v = 0; v < 5; ++v) { idx[v] = core_index(G, lb, v, ci.i, ci.j, k); qq[v] = qin[idx[v]]; }
+        p2c(gamma, R, qq, cons);
+        #pragma unroll
+        for (int t = 0; t < kMaxStages; ++t)
+        {
+            if (t >= C.n || C.c[t] == 0.0) continue;
+            double inc[5];
+            for (int v = 0; v < 5; ++v) inc[v] = C.c[t]*C.r[t][idx[v]];
+            for (int v = 0; v < 5; ++v) cons[v] += inc[v];
+        }
+        c2p(gamma, R, cons, qq);
+        for (int v = 0; v < 5; ++v) qout[idx[v]] = qq[v];
+        push_ghosts(G, push, qout, ci, qq, true);
+        stage_x(G, xs, ci, qq);
+        lds_barrier();
+        flush_x(G, push, qout, xs, lb, k);
+        lds_barrier();
+    }
+}
+
 } // namespace
 
 // Compositions with a specialised kernel, as term codes T0 T1 T2 (0: none);
@@ -896,6 +970,32 @@ void launch_flux_div_fixup(const Grid& g, const double* prims, double* rhs, cons
             hipLaunchKernelGGL(k_flux_fixup, grid, dim3(256), 0, s, g.geom, prims, rhs, f, inc, g.d_start);
     }
     SPD_LAUNCH_CHECK();
+}
+
+bool launch_stage_fixup_spec(const Grid& g, double* qin, double* qout, double* rhs, const spd_flux_desc& f,
+                             const double* const* r, const double* coef, int nr, double gamma, double R,
+                             bool pull, int wall_y, double t_wall, hipStream_t s)
+{
+    if (nr < 1 || nr > kMaxStages) throw ArgError{"stage fix-up: 1..SPD_RK_MAX_STAGES coefficients"};
+    Coefs C{};
+    C.n = nr;
+    for (int j = 0; j < nr; ++j) { C.r[j] = r[j]; C.c[j] = coef[j]; }
+    const dim3 grid((unsigned)g.geom.nblk);
+    const int p = pull ? 1 : 0;
+    switch (flux_div_spec_code(f))
+    {
+#define SPD_STAGE_FIXUP_LAUNCH(a, b, c) \
+        case (a)*10000 + (b)*100 + (c): \
+            hipLaunchKernelGGL((k_stage_fixup_spec<a, b, c>), grid, dim3(256), 0, s, g.geom, qin, qout, rhs, f, C, \
+                               gamma, R, g.d_push, g.d_nbr, g.d_start, p, wall_y, t_wall); \
+            break;
+        SPD_SPEC_FAST_LIST(SPD_STAGE_FIXUP_LAUNCH)
+#undef SPD_STAGE_FIXUP_LAUNCH
+        default:
+            return false;
+    }
+    SPD_LAUNCH_CHECK();
+    return true;
 }
 
 } // namespace spd

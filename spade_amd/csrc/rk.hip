@@ -28,8 +28,6 @@ __device__ inline bool interior_cell(const DevGeom& G, CellIt& c)
     return true;
 }
 
-constexpr int kMaxStages = SPD_RK_MAX_STAGES;
-struct Coefs { const double* r[kMaxStages]; double c[kMaxStages]; int n; };
 
 __global__ void __launch_bounds__(256)
 k_rk_transform(DevGeom G, double* __restrict__ q, Coefs C, double gamma, double R, const int* __restrict__ push)

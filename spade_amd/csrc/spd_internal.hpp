@@ -144,6 +144,14 @@ int flux_div_spec_code(const spd_flux_desc& f);
 // the composition contains a hybrid_scheme_t term
 bool flux_has_hybrid(const spd_flux_desc& f);
 void launch_exchange(const Grid& g, double* a, hipStream_t s);
+// The fix-up of a fused stage in one launch for the compositions with a
+// specialised per-cell kernel: per block with d_start[lb] < nz, (pull) its
+// ghosts and (wall_y) wall slabs, the residual of its planes from there into
+// rhs, and their stage qin -> qout with r[0..nr-1], r[nr-1] = rhs; false
+// (nothing launched) for the other compositions
+bool launch_stage_fixup_spec(const Grid& g, double* qin, double* qout, double* rhs, const spd_flux_desc& f,
+                             const double* const* r, const double* coef, int nr, double gamma, double R,
+                             bool pull, int wall_y, double t_wall, hipStream_t s);
 // the exchange of the blocks with d_start[lb] < nz only (the others exit at
 // once); wall_y: those at a y wall then fill its isothermal ghost slab at t_wall
 void launch_exchange_flagged(const Grid& g, double* a, hipStream_t s, int wall_y = 0, double t_wall = 0.0);

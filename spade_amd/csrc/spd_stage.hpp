@@ -1,7 +1,8 @@
 // Per-cell pieces of a transform-aware RK stage shared by the stage kernels
-// (rk.hip) and the fused flux + stage kernel (flux_fast.hip): the prim<->cons
-// transforms, and the fused exchange that writes a stage result into the
-// ghost cells of the neighbour blocks.  Compiled without contraction, so
+// (rk.hip), the fused flux + stage kernel (flux_fast.hip) and its fix-up
+// (flux_generic.hip): the prim<->cons transforms, the fused exchange that
+// writes a stage result into the ghost cells of the neighbour blocks, the
+// exchange of one block's ghost cells and the isothermal wall mirror.  Compiled without contraction, so
 // both kernels round every expression alike.
 #pragma once
 #include "spd_internal.hpp"
@@ -154,6 +155,128 @@ __device__ inline void push_result(const DevGeom& G, const int* __restrict__ pus
     stage_x(G, xs, c, qq);
     lds_barrier();
     flush_x(G, push, q, xs, c.lb, c.k);
+}
+
+constexpr int kMaxStages = SPD_RK_MAX_STAGES;
+// stage coefficients: qout = c2p(p2c(qin) + c[0] r[0] + .. + c[n-1] r[n-1])
+struct Coefs { const double* r[kMaxStages]; double c[kMaxStages]; int n; };
+
+// Ghost exchange (grid/exchange_pattern.h:382-424): every ghost cell copies
+// the cell of the neighbour block that owns it (26-neighbour table built on
+// the host, grid/cartesian_blocks.h:73-105; region (ex+1) + 3(ey+1) + 9(ez+1),
+// e_d = -1 below the interior, +1 above, 0 inside; -1 = not exchanged).  The
+// items of a block, in storage order, one per thread (grid: items / 256 x
+// blocks; every thread's copy is independent of the others', so the loads of
+// all items are in flight together):
+//   (i) the ghost rows of the core part -- rows (j, k) with j or k outside
+//       the interior: the 2 ng planes below / above in k, then the 2 ng rows
+//       below / above in j of each interior plane -- one double of a row per
+//       item, so a wave copies whole rows (nc0 contiguous doubles);
+//  (ii) the x-halo: one item per (side, row) copies the row's ng x-ghost
+//       cells on that side, whose sources are adjacent interior cells of one
+//       row of the x-neighbour (i = -1-l <- nc0-1-l, i = nc0+l <- l);
+//       consecutive items write consecutive doubles of a section.
+__host__ __device__ inline int exchange_items(const DevGeom& G)
+{
+    return (2*G.ng*G.N[1] + 2*G.ng*G.nc[2])*G.nc[0] + 2*G.N[1]*G.N[2];
+}
+
+// Exchange item t of block lb (storage order of its ghost cells, see above)
+__device__ inline void exchange_item(const DevGeom& G, const int* __restrict__ nbr, double* __restrict__ a,
+                                     int lb, int t)
+{
+    const int n0 = G.nc[0], n1 = G.nc[1], n2 = G.nc[2], ng = G.ng;
+    const int N1 = G.N[1];
+    const int rows = N1*G.N[2];
+    const long long sv = G.sv;
+    const int* nb = nbr + 27*lb;
+    double* __restrict__ blk = a + (long long)lb*G.sb;
+    auto region = [](int x, int n) { return x < 0 ? -1 : (x >= n ? 1 : 0); };
+
+    const int plane_rows = 2*ng*N1;
+    const int core_items = (plane_rows + 2*ng*n2)*n0;
+    if (t < core_items)
+    {
+        const int i = t % n0, g = t / n0;
+        int j, k;
+        if (g < plane_rows)
+        {
+            const int kk = g / N1;
+            j = g - kk*N1 - ng;
+            k = kk < ng ? kk - ng : n2 + kk - ng;
+        }
+        else
+        {
+            const int h = g - plane_rows;
+            const int jj = h % (2*ng);
+            k = h / (2*ng);
+            j = jj < ng ? jj - ng : n1 + jj - ng;
+        }
+        const int ey = region(j, n1), ez = region(k, n2);
+        const int src = nb[1 + 3*(ey + 1) + 9*(ez + 1)];
+        if (src < 0) return;
+        const int r = (k + ng)*N1 + (j + ng);
+        const int sr = (k - ez*n2 + ng)*N1 + (j - ey*n1 + ng);
+        const double* s = a + (long long)src*G.sb + (long long)sr*n0 + i;
+        double* d = blk + (long long)r*n0 + i;
+        double q[5];
+        #pragma unroll
+        for (int v = 0; v < 5; ++v) q[v] = s[v*sv];
+        #pragma unroll
+        for (int v = 0; v < 5; ++v) d[v*sv] = q[v];
+        return;
+    }
+    const int x = t - core_items;
+    if (x >= 2*rows) return;
+    const int side = x / rows, r = x - side*rows;
+    const int j = r % N1 - ng, k = r / N1 - ng;
+    const int ex = side ? 1 : -1, ey = region(j, n1), ez = region(k, n2);
+    const int src = nb[(ex + 1) + 3*(ey + 1) + 9*(ez + 1)];
+    if (src < 0) return;
+    const int sr = (k - ez*n2 + ng)*N1 + (j - ey*n1 + ng);
+    const double* s = a + (long long)src*G.sb + (long long)sr*n0;
+    for (int l = 0; l < ng; ++l)
+    {
+        const int si = side ? l : n0 - 1 - l;
+        double* d = blk + G.xh + (long long)(2*l + side)*rows + r;
+        double q[5];
+        #pragma unroll
+        for (int v = 0; v < 5; ++v) q[v] = s[si + v*sv];
+        #pragma unroll
+        for (int v = 0; v < 5; ++v) d[v*sv] = q[v];
+    }
+}
+
+// grid (workgroups per block) x blocks.  With xcd_runs the linear workgroup
+// number is permuted so that each XCD takes a contiguous run of blocks: a
+// block's two x-neighbours both gather x-halo cells from its rows (16 bytes
+// of each 128-byte row for each of them), and in runs they do so on the same
+// XCD at about the same time, so the second gather finds the lines in L2
+// boundary_fill with the isothermal no-slip mirror (grid/boundary_fill.h:
+// 104-130): ghost (normal index < 0 or >= n, tangential over the full stored
+// range) = (p, 2 Tw - T, -u, -v, -w) of the mirror cell.  Cell c of the ghost
+// slab of face (idir, pm) of block lb.
+__device__ inline void wall_cell(const DevGeom& G, double* __restrict__ a, int lb, int idir, int pm, int c,
+                                 double t_wall)
+{
+    int ext[3] = {G.N[0], G.N[1], G.N[2]};
+    ext[idir] = G.ng;
+    if (c >= ext[0]*ext[1]*ext[2]) return;
+    int idx[3] = {c % ext[0] - G.ng, (c / ext[0]) % ext[1] - G.ng, c / (ext[0]*ext[1]) - G.ng};
+    if (pm == 0) idx[idir] = -G.ng + (idx[idir] + G.ng);                 // [-ng, 0)
+    else         idx[idir] = G.nc[idir] + (idx[idir] + G.ng);            // [n, n+ng)
+    int im[3] = {idx[0], idx[1], idx[2]};
+    im[idir] = (pm == 0) ? (-1 - idx[idir]) : (2*G.nc[idir] - (idx[idir] + 1));
+    const double p = a[cell_index(G, lb, 0, im[0], im[1], im[2])];
+    const double T = a[cell_index(G, lb, 1, im[0], im[1], im[2])];
+    const double u = a[cell_index(G, lb, 2, im[0], im[1], im[2])];
+    const double v = a[cell_index(G, lb, 3, im[0], im[1], im[2])];
+    const double w = a[cell_index(G, lb, 4, im[0], im[1], im[2])];
+    a[cell_index(G, lb, 0, idx[0], idx[1], idx[2])] = p;
+    a[cell_index(G, lb, 1, idx[0], idx[1], idx[2])] = 2.0*t_wall - T;
+    a[cell_index(G, lb, 2, idx[0], idx[1], idx[2])] = -u;
+    a[cell_index(G, lb, 3, idx[0], idx[1], idx[2])] = -v;
+    a[cell_index(G, lb, 4, idx[0], idx[1], idx[2])] = -w;
 }
 
 } // namespace spd

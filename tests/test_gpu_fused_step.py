@@ -124,6 +124,23 @@ def test_fused_with_pow_fixup_equals_separate():
     assert np.array_equal(a, b)
 
 
+def test_fused_merged_fixup_equals_three_launches():
+    """The fix-up of a fused stage in one launch (ghost pull, per-cell
+    residual and stage per flagged block) equals its three separate launches
+    (SPD_SPLIT_FIXUP=1), on a state with out-of-range face temperatures."""
+    sa = _sa()
+    make = _tgv((4, 4, 4), (16, 16, 16), hot=True)
+    print("fused: merged vs three-launch fix-up", flush=True)
+    a = _run(sa, make, (1, 3), True)
+    os.environ["SPD_SPLIT_FIXUP"] = "1"
+    try:
+        b = _run(sa, make, (1, 3), True)
+    finally:
+        os.environ.pop("SPD_SPLIT_FIXUP", None)
+    assert np.isfinite(a).all()
+    assert np.array_equal(a, b)
+
+
 @pytest.mark.parametrize("steps,hot", [((2, 1), False), ((1, 3), False), ((1, 2), True)],
                          ids=["2x1", "1x3", "1x2_fixup"])
 def test_fused_channel_equals_separate(steps, hot):
